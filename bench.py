@@ -1,0 +1,218 @@
+"""Benchmark of the sandengine grid-update step on one MI355X.
+
+Default workload: one 16384x16384 world (u32 cells, 1 GiB per buffer), the
+four-material rule set, seed 1, bottom 30% of rows stone and the rest i.i.d.
+60% EMPTY / 25% sand / 15% water; --steps timed steps after --warmup untimed
+ones.  Both step paths run from the same initial grid:
+  plain   : one HBM pass per step (se_step_plain)
+  blocked : LDS temporal blocking, several sub-steps per launch (se_step_lds_*)
+and their final grids must be identical (checked after the timed region).
+
+Prints ONE JSON line: value = whole-job Gcell-updates/s of the faster path,
+roofline of that path's kernel (algorithmic bytes: 8 B per cell per launch --
+one u32 read and one u32 write), per-step latency p50/p99 for both paths, and
+the CPU baseline (the reference's GLSL program under Mesa llvmpipe on a bounded
+sample, or the oracle's C restatement where the llvmpipe build is absent).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (side, ruleset, seed, ids, percents, stone fraction (num, den), stone id)
+    "single16k": (16384, "four_materials", 1, [0, 3, 4], [60, 25, 15], (3, 10), 5),
+    "single256": (256, "four_materials", 0, [0, 3, 4, 5], [50, 25, 15, 10], (0, 1), 0),
+    "rich8k": (8192, "twelve_materials", 2, [0] + list(range(3, 15)), [40] + [5] * 12, (0, 1), 0),
+}
+DESCR = {
+    "single16k": "single world 16384x16384 u32, 4 materials, seed 1, bottom 30% stone, 60/25/15 empty/sand/water",
+    "single256": "single world 256x256 u32, 4 materials, seed 0, 50/25/15/10 empty/sand/water/stone",
+    "rich8k": "single world 8192x8192 u32, 12 materials with SET/RAND, seed 2, 40% empty + 12x5%",
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_world(S, ctx, rs, wl):
+    side, _, seed, ids, pct, (num, den), stone = WORKLOADS[wl]
+    w = S.World(ctx, rs, side, side)
+    w.frame = 1  # the clearing frame has run; the seeded grid starts at frame 2
+    w.fill_seeded(seed, ids, pct, stone_rows=(side * num) // den, stone_id=stone)
+    return w
+
+
+def time_path(S, w, mode, steps, warmup, sub):
+    adv = (lambda n: w.step(n)) if mode == "plain" else (lambda n: w.step_blocked(n, sub))
+    adv(warmup)
+    w.sync()
+    t0 = time.perf_counter()
+    gpu_ms = w.step_timed(steps, 0 if mode == "plain" else 1, sub)
+    wall = time.perf_counter() - t0
+    return wall, gpu_ms
+
+
+def latencies(w, mode, sub, n):
+    """Per-step latency (ms): each sample is one synchronous step (or one blocked
+    launch divided by its sub-steps)."""
+    out = []
+    for _ in range(n):
+        if mode == "plain":
+            out.append(w.step_timed(1, 0, 0))
+        else:
+            out.append(w.step_timed(sub, 1, sub) / sub)
+    return np.percentile(out, 50), np.percentile(out, 99)
+
+
+def cpu_baseline(wl, side=1024, steps=12, warm=2):
+    """Bounded sample of the same workload on the host cores, in a child process."""
+    code = f"""
+import json, os, sys, time
+sys.path.insert(0, {ROOT!r})
+import numpy as np
+from oracle import lang_ref as L, oracle as O, llvmpipe as LP
+side, steps, warm = {side}, {steps}, {warm}
+_, rsname, seed, ids, pct, (num, den), stone = {WORKLOADS[wl]!r}
+text = open(os.path.join({ROOT!r}, 'sandengine_amd', 'rulesets', rsname + '.yaml')).read()
+res = L.parse_string(text)
+g = O.fill(seed, side, side, ids, pct, (side * num) // den, stone)
+cores = min(16, os.cpu_count() or 1)
+if LP.available():
+    LP.lib(cores)
+    w = LP.World(LP.program_for(res), side, side)
+    w.run(1); w.upload(g)
+    w.run(warm)
+    dt = w.time_run(steps)
+    kind, extra = 'reference', LP.info()
+else:
+    blob = O.compile_ruleset(res)
+    os.environ['OMP_NUM_THREADS'] = str(cores)
+    g = O.run(blob, g, 2, warm)
+    t = time.perf_counter(); O.run(blob, g, 2 + warm, steps); dt = time.perf_counter() - t
+    kind, extra = 'port', 'oracle C restatement (OpenMP)'
+print(json.dumps(dict(value=side * side * steps / dt / 1e9, unit='Gcell-updates/s', cores=cores, kind=kind,
+                      sample=f'{{side}}x{{side}} world of the same fill and rules, {{steps}} timed steps after '
+                             f'{{warm}} warm-up; {{extra}}')))
+"""
+    env = dict(os.environ)
+    env["LP_NUM_THREADS"] = str(min(16, os.cpu_count() or 1))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=600)
+    if r.returncode != 0:
+        log("cpu baseline failed:", r.stderr[-2000:])
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def load_traffic(kernel_prefix):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), from profiles/traffic.json."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    d = json.load(open(p))
+    for k, v in d.get("kernels", {}).items():
+        if k.startswith(kernel_prefix):
+            return v.get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=24)
+    ap.add_argument("--workload", default="single16k", choices=sorted(WORKLOADS))
+    ap.add_argument("--sub", type=int, default=8, help="sub-steps per LDS launch")
+    ap.add_argument("--paths", default="plain,blocked")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--latency-samples", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import sandengine_amd as S
+    ctx = S.Context(0)
+    side, rsname = WORKLOADS[args.workload][:2]
+    t0 = time.perf_counter()
+    rs = S.RuleSet(ctx, S.bundled_ruleset(rsname))
+    log(f"ruleset {rsname}: JIT {time.perf_counter() - t0:.2f}s")
+    cells = side * side
+    results, worlds = {}, {}
+    for mode in args.paths.split(","):
+        w = make_world(S, ctx, rs, args.workload)
+        wall, gpu_ms = time_path(S, w, mode, args.steps, args.warmup, args.sub)
+        launches = args.steps if mode == "plain" else -(-args.steps // args.sub)
+        avg_launch_ms = gpu_ms / launches
+        r = dict(gcups=cells * args.steps / wall / 1e9, gcups_gpu=cells * args.steps / (gpu_ms / 1e3) / 1e9,
+                 ms_per_step=wall * 1e3 / args.steps, avg_launch_ms=avg_launch_ms, launches=launches,
+                 steps_per_launch=1 if mode == "plain" else args.sub)
+        r["achieved_gbs"] = 8.0 * cells / (avg_launch_ms / 1e3) / 1e9
+        p50, p99 = latencies(w, mode, args.sub, args.latency_samples)
+        r["step_latency_ms"] = {"p50": p50, "p99": p99}
+        log(f"{mode}: {r['gcups']:.1f} Gcell-upd/s wall, {r['gcups_gpu']:.1f} by events, "
+            f"{r['avg_launch_ms']:.3f} ms/launch, {r['achieved_gbs']:.0f} GB/s algorithmic")
+        results[mode] = r
+        worlds[mode] = w
+    check = None
+    if not args.no_check and "plain" in worlds and "blocked" in worlds:
+        # both advanced warmup + steps + latency steps from the same grid; align frames
+        a, b = worlds["plain"], worlds["blocked"]
+        if a.frame < b.frame:
+            a.step(b.frame - a.frame)
+        elif b.frame < a.frame:
+            b.step(a.frame - b.frame)
+        ga = a.download()
+        gb = b.download()
+        check = dict(identical=bool(np.array_equal(ga, gb)), frame=a.frame)
+        del ga, gb
+        log(f"plain vs blocked after frame {a.frame}: identical={check['identical']}")
+    for w in worlds.values():
+        w.close()
+    best = max(results, key=lambda m: results[m]["gcups"])
+    R = results[best]
+    kernel = "se_step_plain" if best == "plain" else "se_step_lds"
+    traffic = load_traffic(kernel)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload)
+    out = {
+        "metric": "Gcell-updates/s & HBM-roofline %; p50/p99 step latency, batched vs plain",
+        "value": R["gcups"],
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": R["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded SplitMix64 fill generated on the device)",
+        "config": {"workload": DESCR[args.workload], "path": best, "rules": rsname,
+                   "steps_per_launch": R["steps_per_launch"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": R["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": R["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": 8 * cells, "avg_launch_ms": R["avg_launch_ms"]},
+        "cpu_baseline": cpu,
+        "paths": results,
+        "parity": check,
+    }
+    print(json.dumps(out))
+    if check is not None and not check["identical"]:
+        sys.exit(1)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,134 @@
+/*
+ * sandengine_hip.h -- C ABI of the MI355X-native sandengine grid-update step.
+ *
+ * This is the drop-in boundary for the reference's simulation path:
+ *   - the rule-language front end  sandengine-lang (parse_string,
+ *     create_glsl_from_parser) -- sandengine-lang/src/parser.rs:93,
+ *     sandengine-lang/src/lib.rs:17;
+ *   - the step driver  sandengine_core::simulation::Simulation
+ *     (new / run / modifications / params.frame) --
+ *     sandengine-core/src/simulation.rs:97-253.
+ * The reference has no C ABI (it is Rust calling GL through glium); each entry
+ * point below names the reference interface it replaces.
+ *
+ * Conventions: every int-returning function returns SE_OK (0) or a negative
+ * se_status; nothing aborts or unwinds across the boundary; se_last_error()
+ * gives a thread-local message for the last failure.  Host buffers belong to
+ * the caller; device buffers and compiled modules belong to the library.  A
+ * se_world must not be used from two threads at once; a se_ctx is internally
+ * synchronised.  Grids are u32 cell ids, row-major, W*H, y growing down.
+ */
+#ifndef SANDENGINE_HIP_H
+#define SANDENGINE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct se_ctx se_ctx;
+typedef struct se_ruleset se_ruleset;
+typedef struct se_world se_world;
+typedef struct se_batch se_batch;
+
+/* == SimModification, sandengine-core/src/simulation.rs:46-56 (32 bytes).
+ * mod_shape: 0 circle, 1 square (MODSHAPE_*, simulation.rs:41-42). */
+typedef struct {
+    int32_t position[2];
+    int32_t mod_shape;
+    int32_t mod_size;
+    int32_t mod_matID;
+    int32_t _pad4[3];
+} se_modification;
+
+enum se_status {
+    SE_OK = 0,
+    SE_ERR_ARG = -1,     /* bad argument or size */
+    SE_ERR_PARSE = -2,   /* rule file rejected (ParsingErr category in the message) */
+    SE_ERR_CODEGEN = -3, /* rule set uses something the HIP backend cannot express */
+    SE_ERR_JIT = -4,     /* hiprtc compile or module load failed */
+    SE_ERR_HIP = -5,     /* HIP runtime error */
+    SE_ERR_OOM = -6,     /* device or host allocation failed */
+    SE_ERR_FULL = -7,    /* admission refused: HBM budget or slot limit */
+    SE_ERR_STATE = -8    /* call not valid in the current state */
+};
+
+#define SE_MAX_MODIFICATIONS 256 /* MAX_MODIFICATIONS, simulation.rs:43 */
+
+/* Thread-local text of the last error. */
+const char* se_last_error(void);
+const char* se_version(void);
+void se_free(void* p);
+
+/* ---------------------------------------------------------------- language (no GPU needed) */
+/* parse_string (sandengine-lang/src/parser.rs:93-152); writes the IR as JSON
+ * (malloc'd, free with se_free).  On a rule-file error returns SE_ERR_PARSE and
+ * *kind_out (if given) names the ParsingErr category. */
+int se_parse_ir_json(const char* yaml, size_t len, char** json_out, char** kind_out);
+/* create_glsl_from_parser (sandengine-lang/src/lib.rs:17-147): the text of
+ * gen/materials.glsl and gen/rules.glsl, returned instead of written to disk. */
+int se_emit_glsl(const char* yaml, size_t len, char** materials_out, char** rules_out);
+/* HIP backend: the translation unit hiprtc compiles for this rule set. */
+int se_emit_hip(const char* yaml, size_t len, char** src_out);
+/* Compiles the HIP backend's output with hiprtc for gfx950 (no device needed);
+ * *code_bytes receives the code-object size. */
+int se_jit_check(const char* yaml, size_t len, size_t* code_bytes);
+
+/* ---------------------------------------------------------------- context */
+/* One per device; owns the JIT cache and the default stream.
+ * Replaces the GL context + ComputeShader::from_source (simulation.rs:128-137). */
+int se_ctx_create(int device, se_ctx** out);
+void se_ctx_destroy(se_ctx* ctx);
+/* Device HBM bytes (total, free). */
+int se_ctx_mem_info(se_ctx* ctx, size_t* total, size_t* free_bytes);
+
+/* ---------------------------------------------------------------- rule sets */
+/* parse + HIP codegen + hiprtc JIT (gfx950), cached by source hash.  Replaces
+ * parse_string + create_glsl_from_parser + build_compute_shaders + shader
+ * compile (src/lib.rs:9-27, simulation.rs:129-138). */
+int se_ruleset_from_yaml(se_ctx* ctx, const char* yaml, size_t len, se_ruleset** out);
+void se_ruleset_destroy(se_ruleset* rs);
+/* Material id by name (ids follow YAML order after EMPTY 0, NULL 1, WALL 2;
+ * sandengine-lang/src/parser/materials.rs:52-91). */
+int se_ruleset_material_id(const se_ruleset* rs, const char* name, int32_t* id);
+int se_ruleset_material_count(const se_ruleset* rs, int32_t* n);
+
+/* ---------------------------------------------------------------- worlds */
+/* Simulation::new (simulation.rs:128-192): frame 0, every cell EMPTY. */
+int se_world_create(se_ctx* ctx, const se_ruleset* rs, uint32_t w, uint32_t h, se_world** out);
+void se_world_destroy(se_world* w);
+/* Host <-> device copies of the current grid (the reference has none; they make
+ * seeded runs and checks possible).  n must equal w*h. */
+int se_world_upload(se_world* w, const uint32_t* cells, size_t n);
+int se_world_download(se_world* w, uint32_t* cells, size_t n);
+/* Simulation::modifications.push (simulation.rs:125; sandengine-core/src/lib.rs:59-67):
+ * queued for the next step only; more than SE_MAX_MODIFICATIONS queued -> SE_ERR_FULL. */
+int se_world_push_modification(se_world* w, const se_modification* m);
+/* nsteps x Simulation::run (simulation.rs:195-252): each does frame += 1 then
+ * one step with that frame (frame 1 clears); queued modifications apply to the
+ * first of them and are then cleared.  Plain kernel: one pass over HBM per step. */
+int se_world_step(se_world* w, uint32_t nsteps);
+/* Same result as se_world_step, advanced by the LDS temporal-blocked kernel in
+ * launches of up to `sub` sub-steps (0 = autotuned choice). */
+int se_world_step_blocked(se_world* w, uint32_t nsteps, uint32_t sub);
+/* Params.frame (simulation.rs:78, 201). Negative frames are rejected. */
+int se_world_get_frame(const se_world* w, int32_t* frame);
+int se_world_set_frame(se_world* w, int32_t frame);
+int se_world_sync(se_world* w);
+/* Device view of the current grid: pointer, row pitch in cells. */
+int se_world_device_ptr(se_world* w, void** ptr, uint32_t* pitch);
+/* Seeded i.i.d. fill on the device (benchmark workloads): cell (x,y) draws
+ * r = hi32(splitmix64((seed << 40) ^ (y*w + x))) and takes ids[k] for the first k
+ * with r < thresholds[k]; rows y >= h - stone_rows become stone_id. */
+int se_world_fill_seeded(se_world* w, uint64_t seed, const uint32_t* ids, const uint32_t* thresholds, int n,
+                         uint32_t stone_rows, uint32_t stone_id);
+/* Runs nsteps (mode 0 plain, 1 blocked with `sub`) between two HIP events on
+ * the world's stream; *gpu_ms receives their elapsed time. */
+int se_world_step_timed(se_world* w, uint32_t nsteps, int mode, uint32_t sub, float* gpu_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SANDENGINE_HIP_H */

@@ -63,9 +63,30 @@ class ParsingErr(Exception):
 
 def _dbg(v: Any) -> str:
     # Rust `{:?}` of &str / String prints the quoted string.
+    if isinstance(v, _Val):
+        return repr(v)
     if isinstance(v, str):
         return '"' + v.replace("\\", "\\\\").replace('"', '\\"') + '"'
     return repr(v)
+
+
+class _Val:
+    """A serde_yaml Value passed as `wrong_type` (Debug-printed as Number(..) etc.)."""
+
+    def __init__(self, v):
+        self.v = v
+
+    def __repr__(self):
+        v = self.v
+        if v is None:
+            return "Null"
+        if isinstance(v, bool):
+            return f"Bool({'true' if v else 'false'})"
+        if isinstance(v, (int, float)):
+            return f"Number({v})"
+        if isinstance(v, str):
+            return f'String("{v}")'
+        return "Sequence" if isinstance(v, list) else "Mapping"
 
 
 def _missing(field_name, missing_in):
@@ -231,7 +252,7 @@ def _preparse_keys(m: dict, err_name: str) -> List[str]:
     out = []
     for k in m:
         if not isinstance(k, str):
-            raise _invalid(k, err_name, TYPE_HINT_STRING)
+            raise _invalid(_Val(k), err_name, TYPE_HINT_STRING)
         out.append(k)
     return out
 
@@ -241,7 +262,7 @@ def _check_mapping(d, keyname):
         raise _missing(keyname, "Root/ Base level of YAML file")
     v = d[keyname]
     if not isinstance(v, dict):
-        raise _invalid(v, "Root/ Base level of YAML file", TYPE_HINT_MAPPING)
+        raise _invalid(_Val(v), "Root/ Base level of YAML file", TYPE_HINT_MAPPING)
     return v
 
 
@@ -281,7 +302,7 @@ def _parse_rules(rules: dict, type_names, material_names) -> List[SandRule]:
     out = []
     for name, body in rules.items():
         if not isinstance(name, str):
-            raise _invalid(name, "rules", TYPE_HINT_STRING)
+            raise _invalid(_Val(name), "rules", TYPE_HINT_STRING)
         if_conds, do_actions, probs = [], [], []
         _parse_conditionals(body, False, f"rules/{name}", if_conds, do_actions, probs,
                             type_names, material_names)
@@ -409,7 +430,7 @@ def _parse_types(types: dict, rules: List[SandRule], rule_names, type_names) -> 
     idx = len(ts)
     for name, body in types.items():
         if not isinstance(name, str):
-            raise _invalid(name, "types", TYPE_HINT_STRING)
+            raise _invalid(_Val(name), "types", TYPE_HINT_STRING)
         parent = ""
         if _has(body, "inherits"):
             p = _as_str(body["inherits"])
@@ -477,7 +498,7 @@ def _parse_materials(mats: dict, rules: List[SandRule], type_names) -> List[Sand
     idx = len(ms)
     for name, body in mats.items():
         if not isinstance(name, str):
-            raise _invalid(name, "materials", TYPE_HINT_STRING)
+            raise _invalid(_Val(name), "materials", TYPE_HINT_STRING)
         if not _has(body, "type"):
             raise _missing("type", f"materials/{name}")
         mt = _as_str(body["type"])

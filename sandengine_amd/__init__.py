@@ -1,0 +1,408 @@
+"""sandengine_amd: MI355X-native sandengine grid-update step (Python host side).
+
+Mirrors the reference's interfaces over the C ABI in include/sandengine_hip.h:
+
+* ``parse_string(yaml)``    sandengine_lang::parse_string (sandengine-lang/src/parser.rs:93)
+  returning a ``ParsingResult`` with ``rules`` / ``types`` / ``materials``;
+  errors raise ``ParsingErr`` whose ``kind`` is the reference's category
+  (MissingField, InvalidType, NotFound, NotRecognized; parser.rs:44-73).
+* ``create_glsl_from_parser`` sandengine-lang/src/lib.rs:17 (returns the two
+  generated GLSL texts instead of writing gen/*.glsl).
+* ``Simulation``            sandengine_core::simulation::Simulation
+  (sandengine-core/src/simulation.rs:97-253): ``run()`` is one step
+  (frame += 1, then the step; frame 1 clears), ``modifications`` is the list the
+  caller pushes SimModification records onto, applied to the next run() and
+  then cleared, ``params.frame`` is the frame counter.
+
+The compute path is the HIP library libsandengine_hip.so built in this
+directory; there is no CPU fallback: without the library every entry point
+raises ``SandengineError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsandengine_hip.so")
+RULESETS_DIR = os.path.join(HERE, "rulesets")
+
+MODSHAPE_CIRCLE = 0   # simulation.rs:41
+MODSHAPE_SQUARE = 1   # simulation.rs:42
+MAX_MODIFICATIONS = 256  # simulation.rs:43
+
+SE_OK, SE_ERR_ARG, SE_ERR_PARSE, SE_ERR_CODEGEN, SE_ERR_JIT, SE_ERR_HIP, SE_ERR_OOM, SE_ERR_FULL, \
+    SE_ERR_STATE = 0, -1, -2, -3, -4, -5, -6, -7, -8
+
+
+class SandengineError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+class ParsingErr(SandengineError):
+    """Rule-file error; ``kind`` is the reference's ParsingErr category."""
+
+    def __init__(self, kind: str, message: str):
+        super().__init__(SE_ERR_PARSE, message)
+        self.kind = kind
+
+
+class SeModification(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_int32 * 2), ("mod_shape", ctypes.c_int32),
+                ("mod_size", ctypes.c_int32), ("mod_matID", ctypes.c_int32),
+                ("_pad4", ctypes.c_int32 * 3)]
+
+
+assert ctypes.sizeof(SeModification) == 32
+
+_lib = None
+
+
+def lib():
+    """The HIP library; raises if it has not been built (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SandengineError(SE_ERR_STATE, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        c_p, c_sz, c_u32, c_i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int32
+        PP = ctypes.POINTER(ctypes.c_void_p)
+        L.se_last_error.restype = ctypes.c_char_p
+        L.se_version.restype = ctypes.c_char_p
+        L.se_free.argtypes = [c_p]
+        L.se_parse_ir_json.argtypes = [ctypes.c_char_p, c_sz, PP, PP]
+        L.se_emit_glsl.argtypes = [ctypes.c_char_p, c_sz, PP, PP]
+        L.se_emit_hip.argtypes = [ctypes.c_char_p, c_sz, PP]
+        L.se_jit_check.argtypes = [ctypes.c_char_p, c_sz, ctypes.POINTER(c_sz)]
+        L.se_ctx_create.argtypes = [ctypes.c_int, PP]
+        L.se_ctx_destroy.argtypes = [c_p]
+        L.se_ctx_mem_info.argtypes = [c_p, ctypes.POINTER(c_sz), ctypes.POINTER(c_sz)]
+        L.se_ruleset_from_yaml.argtypes = [c_p, ctypes.c_char_p, c_sz, PP]
+        L.se_ruleset_destroy.argtypes = [c_p]
+        L.se_ruleset_material_id.argtypes = [c_p, ctypes.c_char_p, ctypes.POINTER(c_i32)]
+        L.se_ruleset_material_count.argtypes = [c_p, ctypes.POINTER(c_i32)]
+        L.se_world_create.argtypes = [c_p, c_p, c_u32, c_u32, PP]
+        L.se_world_destroy.argtypes = [c_p]
+        L.se_world_upload.argtypes = [c_p, ctypes.POINTER(c_u32), c_sz]
+        L.se_world_download.argtypes = [c_p, ctypes.POINTER(c_u32), c_sz]
+        L.se_world_push_modification.argtypes = [c_p, ctypes.POINTER(SeModification)]
+        L.se_world_step.argtypes = [c_p, c_u32]
+        L.se_world_step_blocked.argtypes = [c_p, c_u32, c_u32]
+        L.se_world_get_frame.argtypes = [c_p, ctypes.POINTER(c_i32)]
+        L.se_world_set_frame.argtypes = [c_p, c_i32]
+        L.se_world_sync.argtypes = [c_p]
+        L.se_world_device_ptr.argtypes = [c_p, PP, ctypes.POINTER(c_u32)]
+        L.se_world_fill_seeded.argtypes = [c_p, ctypes.c_uint64, ctypes.POINTER(c_u32),
+                                           ctypes.POINTER(c_u32), ctypes.c_int, c_u32, c_u32]
+        L.se_world_step_timed.argtypes = [c_p, c_u32, ctypes.c_int, c_u32, ctypes.POINTER(ctypes.c_float)]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != SE_OK:
+        raise SandengineError(rc, lib().se_last_error().decode(errors="replace"))
+
+
+def _take_str(p: ctypes.c_void_p) -> Optional[str]:
+    if not p.value:
+        return None
+    s = ctypes.string_at(p.value).decode()
+    lib().se_free(p)
+    return s
+
+
+# ---------------------------------------------------------------- language layer
+@dataclass
+class SandRule:
+    name: str
+    ruletype: str
+    if_conds: List[str]
+    do_actions: List[str]
+    probabilities: List[float]
+    mirror: bool
+    precondition: Optional[str]
+    used: bool
+
+
+@dataclass
+class SandType:
+    id: int
+    name: str
+    inherits: str
+    children: List[str]
+    base_rules: List[str]
+
+
+@dataclass
+class SandMaterial:
+    id: int
+    name: str
+    mattype: str
+    color: List[float]
+    emission: List[float]
+    selectable: bool
+    density: float
+    extra_rules: List[str]
+
+
+@dataclass
+class ParsingResult:
+    rules: List[SandRule]
+    types: List[SandType]
+    materials: List[SandMaterial]
+    source: str = ""
+    ir: dict = field(default_factory=dict)
+
+    def material_id(self, name: str) -> int:
+        for m in self.materials:
+            if m.name == name:
+                return m.id
+        raise KeyError(name)
+
+
+def parse_string(yaml_text: str) -> ParsingResult:
+    b = yaml_text.encode()
+    out, kind = ctypes.c_void_p(), ctypes.c_void_p()
+    rc = lib().se_parse_ir_json(b, len(b), ctypes.byref(out), ctypes.byref(kind))
+    if rc == SE_ERR_PARSE:
+        k = _take_str(kind) or "Yaml"
+        raise ParsingErr(k, lib().se_last_error().decode(errors="replace"))
+    _check(rc)
+    ir = json.loads(_take_str(out))
+    return ParsingResult([SandRule(**r) for r in ir["rules"]], [SandType(**t) for t in ir["types"]],
+                         [SandMaterial(**m) for m in ir["materials"]], yaml_text, ir)
+
+
+def parse_path(path: str) -> ParsingResult:
+    """sandengine-lang/src/lib.rs:10-13."""
+    with open(path) as f:
+        return parse_string(f.read())
+
+
+def create_glsl_from_parser(result: ParsingResult) -> Tuple[str, str]:
+    """(gen/materials.glsl, gen/rules.glsl) text for a parsed rule file."""
+    b = result.source.encode()
+    m, r = ctypes.c_void_p(), ctypes.c_void_p()
+    _check(lib().se_emit_glsl(b, len(b), ctypes.byref(m), ctypes.byref(r)))
+    return _take_str(m), _take_str(r)
+
+
+def emit_hip(yaml_text: str) -> str:
+    b = yaml_text.encode()
+    s = ctypes.c_void_p()
+    rc = lib().se_emit_hip(b, len(b), ctypes.byref(s))
+    _check(rc)
+    return _take_str(s)
+
+
+def jit_check(yaml_text: str) -> int:
+    """hiprtc-compiles the rule set for gfx950 (no device needed); returns code bytes."""
+    b = yaml_text.encode()
+    n = ctypes.c_size_t()
+    _check(lib().se_jit_check(b, len(b), ctypes.byref(n)))
+    return n.value
+
+
+def bundled_ruleset(name: str) -> str:
+    with open(os.path.join(RULESETS_DIR, name + ".yaml")) as f:
+        return f.read()
+
+
+BUNDLED = ["four_materials", "twelve_materials", "features"]
+
+
+def jit_check_bundled(verbose: bool = False):
+    for n in BUNDLED:
+        size = jit_check(bundled_ruleset(n))
+        if verbose:
+            print(f"hiprtc gfx950 {n}: {size} bytes")
+
+
+# ---------------------------------------------------------------- device runtime
+class Context:
+    """One device: JIT cache + stream (se_ctx)."""
+
+    def __init__(self, device: int = 0):
+        p = ctypes.c_void_p()
+        _check(lib().se_ctx_create(device, ctypes.byref(p)))
+        self._p = p
+        self.device = device
+
+    def mem_info(self) -> Tuple[int, int]:
+        t, f = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(lib().se_ctx_mem_info(self._p, ctypes.byref(t), ctypes.byref(f)))
+        return t.value, f.value
+
+    def close(self):
+        if getattr(self, "_p", None) and self._p.value:
+            lib().se_ctx_destroy(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RuleSet:
+    """Parsed + JIT-compiled rule set (se_ruleset)."""
+
+    def __init__(self, ctx: Context, yaml_text: str):
+        b = yaml_text.encode()
+        p = ctypes.c_void_p()
+        rc = lib().se_ruleset_from_yaml(ctx._p, b, len(b), ctypes.byref(p))
+        if rc == SE_ERR_PARSE:
+            parse_string(yaml_text)  # raises ParsingErr with the category
+        _check(rc)
+        self._p, self.ctx, self.source = p, ctx, yaml_text
+        n = ctypes.c_int32()
+        _check(lib().se_ruleset_material_count(self._p, ctypes.byref(n)))
+        self.nmat = n.value
+
+    def material_id(self, name: str) -> int:
+        v = ctypes.c_int32()
+        _check(lib().se_ruleset_material_id(self._p, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "_p", None) and self._p.value:
+            lib().se_ruleset_destroy(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class World:
+    """One simulated world resident in HBM (se_world)."""
+
+    def __init__(self, ctx: Context, rules: RuleSet, w: int, h: int):
+        p = ctypes.c_void_p()
+        _check(lib().se_world_create(ctx._p, rules._p, w, h, ctypes.byref(p)))
+        self._p, self.ctx, self.rules, self.W, self.H = p, ctx, rules, w, h
+
+    def upload(self, grid: np.ndarray):
+        g = np.ascontiguousarray(grid, dtype=np.uint32)
+        if g.shape != (self.H, self.W):
+            raise ValueError(f"grid shape {g.shape} != {(self.H, self.W)}")
+        _check(lib().se_world_upload(self._p, g.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), g.size))
+
+    def download(self) -> np.ndarray:
+        g = np.empty((self.H, self.W), np.uint32)
+        _check(lib().se_world_download(self._p, g.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), g.size))
+        return g
+
+    def push_modification(self, x: int, y: int, shape: int, size: int, mat: int):
+        m = SeModification()
+        m.position[0], m.position[1] = x, y
+        m.mod_shape, m.mod_size, m.mod_matID = shape, size, mat
+        _check(lib().se_world_push_modification(self._p, ctypes.byref(m)))
+
+    def step(self, n: int = 1):
+        _check(lib().se_world_step(self._p, n))
+
+    def step_blocked(self, n: int = 1, sub: int = 0):
+        _check(lib().se_world_step_blocked(self._p, n, sub))
+
+    def step_timed(self, n: int, mode: int = 0, sub: int = 0) -> float:
+        ms = ctypes.c_float()
+        _check(lib().se_world_step_timed(self._p, n, mode, sub, ctypes.byref(ms)))
+        return ms.value
+
+    @property
+    def frame(self) -> int:
+        v = ctypes.c_int32()
+        _check(lib().se_world_get_frame(self._p, ctypes.byref(v)))
+        return v.value
+
+    @frame.setter
+    def frame(self, f: int):
+        _check(lib().se_world_set_frame(self._p, f))
+
+    def sync(self):
+        _check(lib().se_world_sync(self._p))
+
+    def fill_seeded(self, seed: int, ids: Sequence[int], percents: Sequence[int], stone_rows: int = 0,
+                    stone_id: int = 0):
+        """Device-side seeded fill; percents are integers summing to 100."""
+        if sum(percents) != 100 or len(ids) != len(percents):
+            raise ValueError("percents must sum to 100, one per id")
+        thr, cum = [], 0
+        for p in percents:
+            cum += p
+            thr.append(min((cum << 32) // 100, 0xFFFFFFFF))
+        a = (ctypes.c_uint32 * len(ids))(*ids)
+        t = (ctypes.c_uint32 * len(ids))(*thr)
+        _check(lib().se_world_fill_seeded(self._p, seed, a, t, len(ids), stone_rows, stone_id))
+
+    def close(self):
+        if getattr(self, "_p", None) and self._p.value:
+            lib().se_world_destroy(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- reference-shaped API
+@dataclass
+class SimModification:
+    """sandengine-core/src/simulation.rs:46-56."""
+    position: Tuple[int, int] = (0, 0)
+    mod_shape: int = MODSHAPE_CIRCLE
+    mod_size: int = 0
+    mod_matID: int = 0
+
+
+@dataclass
+class Params:
+    """The parts of simulation.rs:69-91 Params the step reads."""
+    frame: int = 0
+
+
+class Simulation:
+    """Drop-in for sandengine_core::simulation::Simulation over the HIP path."""
+
+    def __init__(self, size: Tuple[int, int], rules, device: int = 0, ctx: Optional[Context] = None,
+                 blocked: bool = False):
+        self.ctx = ctx or Context(device)
+        text = rules.source if isinstance(rules, ParsingResult) else rules
+        self.ruleset = RuleSet(self.ctx, text)
+        self.size = size
+        self.world = World(self.ctx, self.ruleset, size[0], size[1])
+        self.modifications: List[SimModification] = []
+        self.params = Params()
+        self.blocked = blocked
+
+    def run(self):
+        """simulation.rs:195-252: one step; queued modifications apply to it."""
+        w = self.world
+        w.frame = self.params.frame
+        for m in self.modifications[:MAX_MODIFICATIONS]:
+            w.push_modification(m.position[0], m.position[1], m.mod_shape, m.mod_size, m.mod_matID)
+        (w.step_blocked if self.blocked else w.step)(1)
+        self.params.frame = w.frame
+        self.modifications.clear()
+
+    @property
+    def grid(self) -> np.ndarray:
+        return self.world.download()
+
+    def upload(self, grid: np.ndarray):
+        self.world.upload(grid)

@@ -1,0 +1,41 @@
+// HIP backend of the rule language: IR -> CDNA4 kernel source for hiprtc.
+//
+// The rule chain the GLSL backend expresses as rule_* functions
+// (sandengine-lang/src/parser/rules.rs:47-97, lib.rs:78-147) becomes one
+// straight-line device function se_block() over four u32 material ids:
+//   * conditions are parsed from the same GLSL text the reference emits
+//     (including the unparenthesised "rand.y <= p && " prefix, rules.rs:55-62);
+//   * every sub-expression that depends on one cell only is folded at code
+//     generation time into a material bitmask test;
+//   * density comparisons between cells become comparisons of density ranks;
+//   * rand comparisons against constants become integer thresholds on the
+//     hash lane (exact: rand = RNE_f32(h) * 2^-32, math.glsl:74-79).
+#pragma once
+#include <array>
+#include <string>
+#include <vector>
+
+#include "se_lang.hpp"
+
+namespace se {
+
+struct LdsVariant {
+    int tw, th, hb;
+};
+
+struct HipGenInfo {
+    int nmat = 0;
+    std::vector<int> rand_lanes;  // hash lanes the rule chain reads besides x
+    int nrules = 0;
+};
+
+// Generated device code for the rule set (SE_NMAT, se_fixed, se_block, ...).
+std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info = nullptr);
+
+// Full translation unit: rules + kernel templates + LDS instantiations.
+std::string hip_program_source(const ParsingResult& r, const std::vector<LdsVariant>& lds);
+
+// Name of an LDS kernel instantiation.
+std::string lds_kernel_name(const LdsVariant& v);
+
+}  // namespace se

@@ -1,0 +1,318 @@
+// Kernel templates for one sandengine grid step on CDNA4 (gfx950).
+//
+// This text is appended to the per-rule-set code the HIP backend generates
+// (se_hipgen.cpp: SE_NMAT, se_fixed(), se_block(...)) and JIT-compiled with
+// hiprtc.  It is not compiled on its own.
+//
+// Grid layout in HBM: u32 cell ids, row-major, row pitch `pitch` cells
+// (a multiple of 64, so every row starts 256-byte aligned); x >= W is padding
+// that is never read as a cell.  Out-of-bounds cells read as WALL
+// (shaders/compute/operations.glsl:45-51 with SCREEN_IS_BORDER); ids >= SE_NMAT
+// decode to NULL (gen/materials.glsl:79-86).
+//
+// The step: Margolus 2x2 blocks with origin 2*i - off, off = offset(frame % 4)
+// (operations.glsl:25-34); one evaluation per block is exactly equivalent to
+// the reference's one invocation per cell (falling_sand.glsl:61-118, each
+// invocation recomputes its block and keeps one cell).
+typedef unsigned int u32;
+typedef unsigned long long u64;
+
+#define SE_EMPTY 0u
+#define SE_NULLID 1u
+#define SE_WALLID 2u
+
+struct se_mod {  // == SimModification (sandengine-core/src/simulation.rs:46-56), 32 B
+    int x, y, shape, size, mat, pad0, pad1, pad2;
+};
+
+__device__ __forceinline__ u32 se_decode(u32 v) { return v < SE_NMAT ? v : SE_NULLID; }
+
+// sqrt_rn(s) <= R for f32 s >= 0, exactly: the round-to-nearest square root is
+// <= R iff sqrt(s) < R + ulp(R)/2 (no ties are possible for float s), and that
+// midpoint squared is exact in double.
+__device__ __forceinline__ bool se_sqrt_le(float s, float R) {
+    if (!(R >= 0.0f)) return false;
+    if (R == __builtin_inff()) return true;
+    const float up = __builtin_bit_cast(float, __builtin_bit_cast(unsigned int, R) + 1u);  // next float above R
+    double m = 0.5 * ((double)R + (double)up);
+    return (double)s < m * m;
+}
+
+// Modification override for cell (x, y): returns the id or -1
+// (falling_sand.glsl:134-179: last matching entry wins, a NULL material hides
+// earlier matches, the list was cut at the first size-0 entry by the host).
+__device__ __forceinline__ int se_mod_override(const se_mod* __restrict__ mods, int nmods, int x, int y) {
+    int fin = -1;
+    for (int i = 0; i < nmods; i++) {
+        const int mx = mods[i].x, my = mods[i].y, sz = mods[i].size, sh = mods[i].shape;
+        int dx = mx - x, dy = my - y;
+        dx = dx < 0 ? -dx : dx;
+        dy = dy < 0 ? -dy : dy;
+        bool hit;
+        if (sh == 0) {
+            float fx = (float)dx, fy = (float)dy;
+            float s = __fadd_rn(__fmul_rn(fx, fx), __fmul_rn(fy, fy));
+            hit = se_sqrt_le(s, (float)sz);
+        } else if (sh == 1) {
+            hit = dx <= sz && dy <= sz;
+        } else {
+            hit = false;
+        }
+        if (hit) {
+            const int m = mods[i].mat;
+            fin = (m >= 0 && (u32)m < SE_NMAT) ? m : (int)SE_NULLID;
+        }
+    }
+    return fin == (int)SE_NULLID ? -1 : fin;
+}
+
+// Does any modification's bounding box touch rows [ya, yb] x columns [xa, xb]?
+__device__ __forceinline__ bool se_mods_touch(const se_mod* __restrict__ mods, int nmods, int xa, int xb, int ya,
+                                              int yb) {
+    for (int i = 0; i < nmods; i++) {
+        const long r = mods[i].size < 0 ? 0 : mods[i].size;
+        const long mx = mods[i].x, my = mods[i].y;
+        if (mx + r >= xa && mx - r <= xb && my + r >= ya && my - r <= yb) return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ u32 se_cell_at(const u32* __restrict__ row, int x, int W) {
+    return x < W ? se_decode(row[x]) : SE_WALLID;
+}
+
+// ---------------------------------------------------------------------------
+// Plain kernel: one wavefront per (256-cell strip, block row); each lane owns an
+// aligned quad of 4 cells in both rows of the block row and loads/stores it with
+// 16-byte accesses.  With x offset 0 the lane's two blocks lie inside its quad.
+// With x offset 1 the lane evaluates the blocks at x0+1 and x0+3; the cell x0+4
+// comes from lane+1 and the new value of x0 from lane-1 (cross-lane shuffles),
+// lane 63 loads/stores the one cell past the strip, and lane 0 of strip 0
+// evaluates the edge block at x = -1.  Every block is evaluated exactly once.
+// Reads `in`, writes `out` (ping-pong).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint4 se_load_quad(const u32* __restrict__ row, int x0, int W) {
+    uint4 q = make_uint4(SE_WALLID, SE_WALLID, SE_WALLID, SE_WALLID);
+    if (row != nullptr && x0 < W) {
+        uint4 v = *reinterpret_cast<const uint4*>(row + x0);
+        q.x = se_decode(v.x);
+        q.y = x0 + 1 < W ? se_decode(v.y) : SE_WALLID;
+        q.z = x0 + 2 < W ? se_decode(v.z) : SE_WALLID;
+        q.w = x0 + 3 < W ? se_decode(v.w) : SE_WALLID;
+    }
+    return q;
+}
+
+__device__ __forceinline__ void se_apply_mods_quad(uint4& q, const se_mod* __restrict__ mods, int nmods, int x0,
+                                                   int y) {
+    int o;
+    if ((o = se_mod_override(mods, nmods, x0, y)) >= 0) q.x = (u32)o;
+    if ((o = se_mod_override(mods, nmods, x0 + 1, y)) >= 0) q.y = (u32)o;
+    if ((o = se_mod_override(mods, nmods, x0 + 2, y)) >= 0) q.z = (u32)o;
+    if ((o = se_mod_override(mods, nmods, x0 + 3, y)) >= 0) q.w = (u32)o;
+}
+
+extern "C" __global__ __launch_bounds__(256) void se_step_plain(
+    const u32* __restrict__ in, u32* __restrict__ out, int W, int H, int pitch, int frame, int offx, int offy,
+    int strips, int brows, const se_mod* __restrict__ mods, int nmods) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    const int strip = wave % strips;
+    const int brow = wave / strips;
+    if (brow >= brows) return;  // whole wavefront leaves together
+    const int ya = 2 * brow - offy, yb = ya + 1;
+    const int base = strip * 256;
+    const int x0 = base + 4 * lane;
+    const u32* rowa = (ya >= 0 && ya < H) ? in + (size_t)ya * pitch : nullptr;
+    const u32* rowb = (yb < H) ? in + (size_t)yb * pitch : nullptr;
+    uint4 a = se_load_quad(rowa, x0, W);
+    uint4 b = se_load_quad(rowb, x0, W);
+    const u32 fr = (u32)frame;
+    if (offx == 0) {
+        se_block(a.x, a.y, b.x, b.y, (u32)x0, (u32)ya, fr);
+        se_block(a.z, a.w, b.z, b.w, (u32)(x0 + 2), (u32)ya, fr);
+    } else {
+        // cell x0+4 of both rows: lane+1's first cell; lane 63 loads it
+        u32 ea = __shfl_down(a.x, 1, 64), eb = __shfl_down(b.x, 1, 64);
+        if (lane == 63) {
+            const int xe = base + 256;
+            ea = rowa ? se_cell_at(rowa, xe, W) : SE_WALLID;
+            eb = rowb ? se_cell_at(rowb, xe, W) : SE_WALLID;
+        }
+        u32 edge_a = 0, edge_b = 0;
+        if (strip == 0 && lane == 0) {  // block at x = -1: left column is WALL
+            u32 w0 = SE_WALLID, w2 = SE_WALLID;
+            edge_a = a.x;
+            edge_b = b.x;
+            se_block(w0, edge_a, w2, edge_b, 0xFFFFFFFFu, (u32)ya, fr);
+        }
+        se_block(a.y, a.z, b.y, b.z, (u32)(x0 + 1), (u32)ya, fr);
+        se_block(a.w, ea, b.w, eb, (u32)(x0 + 3), (u32)ya, fr);
+        // new value of cell x0 comes from lane-1's second block
+        u32 pa = __shfl_up(ea, 1, 64), pb = __shfl_up(eb, 1, 64);
+        if (lane == 0) {
+            pa = edge_a;
+            pb = edge_b;
+        }
+        if (lane == 63 && base + 256 < W) {  // the cell past the strip
+            const int xe = base + 256;
+            u32 va = ea, vb = eb;
+            if (nmods > 0) {
+                int o;
+                if ((o = se_mod_override(mods, nmods, xe, ya)) >= 0) va = (u32)o;
+                if ((o = se_mod_override(mods, nmods, xe, yb)) >= 0) vb = (u32)o;
+            }
+            if (ya >= 0) out[(size_t)ya * pitch + xe] = va;
+            if (yb < H) out[(size_t)yb * pitch + xe] = vb;
+        }
+        a.x = pa;
+        b.x = pb;
+        if (lane == 0 && strip != 0) {
+            // cell x0 belongs to the previous strip's last block: store x0+1..x0+3
+            if (nmods > 0 && se_mods_touch(mods, nmods, x0, x0 + 3, ya, yb)) {
+                se_apply_mods_quad(a, mods, nmods, x0, ya);
+                se_apply_mods_quad(b, mods, nmods, x0, yb);
+            }
+            if (x0 < W) {
+                if (ya >= 0) {
+                    u32* p = out + (size_t)ya * pitch + x0;
+                    p[1] = a.y; p[2] = a.z; p[3] = a.w;
+                }
+                if (yb < H) {
+                    u32* p = out + (size_t)yb * pitch + x0;
+                    p[1] = b.y; p[2] = b.z; p[3] = b.w;
+                }
+            }
+            return;
+        }
+    }
+    if (x0 >= W) return;
+    if (nmods > 0 && se_mods_touch(mods, nmods, base, base + 256, ya, yb)) {
+        se_apply_mods_quad(a, mods, nmods, x0, ya);
+        se_apply_mods_quad(b, mods, nmods, x0, yb);
+    }
+    if (ya >= 0) *reinterpret_cast<uint4*>(out + (size_t)ya * pitch + x0) = a;
+    if (yb < H) *reinterpret_cast<uint4*>(out + (size_t)yb * pitch + x0) = b;
+}
+
+// Frame 1 of a world clears every cell to EMPTY (falling_sand.glsl:128-131).
+extern "C" __global__ __launch_bounds__(256) void se_fill(u32* __restrict__ g, size_t n, u32 v) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) g[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// LDS temporal-blocked kernel: a workgroup stages a (TW + 2*HB) x (TH + 2*HB)
+// tile (u8 ids) in LDS and advances it `nsub` <= HB Margolus sub-steps in place
+// (blocks partition the cells, so in-place is race-free); each sub-step
+// evaluates only blocks whose four cells are still valid, so the valid region
+// shrinks by at most one cell per side per sub-step and the TW x TH interior is
+// exact after HB sub-steps.  Cells outside the world are WALL in LDS and are
+// never written.  Modifications apply to sub-step 0 only.  Reads `in`, writes
+// the interior to `out` (ping-pong).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void se_offset(u32 f, int& ox, int& oy) {
+    const u32 m = f & 3u;  // frame % 4 for frame >= 0
+    ox = (m == 1u || m == 3u) ? 1 : 0;
+    oy = (m == 1u || m == 2u) ? 1 : 0;
+}
+
+template <int TW, int TH, int HB>
+__device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __restrict__ out, int W, int H,
+                                            int pitch, int frame0, int nsub, int tiles_x,
+                                            const se_mod* __restrict__ mods, int nmods, unsigned char* lds) {
+    constexpr int LW = TW + 2 * HB, LH = TH + 2 * HB;
+    const int tid = threadIdx.x;
+    const int tx = (int)blockIdx.x % tiles_x, ty = (int)blockIdx.x / tiles_x;
+    const int gx0 = tx * TW - HB, gy0 = ty * TH - HB;
+    const bool interior = gx0 >= 0 && gy0 >= 0 && gx0 + LW <= W && gy0 + LH <= H;
+    for (int i = tid; i < LW * LH; i += 256) {
+        const int ly = i / LW, lx = i - ly * LW;
+        const int gx = gx0 + lx, gy = gy0 + ly;
+        u32 v = SE_WALLID;
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) v = se_decode(in[(size_t)gy * pitch + gx]);
+        lds[i] = (unsigned char)v;
+    }
+    __syncthreads();
+    for (int s = 0; s < nsub; s++) {
+        const u32 f = (u32)(frame0 + s);
+        int ox, oy;
+        se_offset(f, ox, oy);
+        const int fx = s + ((s - ox) & 1), fy = s + ((s - oy) & 1);
+        const int nbx = (LW - s - fx) / 2, nby = (LH - s - fy) / 2;  // blocks with fx <= lx, lx+1 <= LW-s-1
+        const int nb = nbx * nby;
+        const bool use_mods = s == 0 && nmods > 0;
+        for (int b = tid; b < nb; b += 256) {
+            const int by = b / nbx, bx = b - by * nbx;
+            const int lx = fx + 2 * bx, ly = fy + 2 * by;
+            unsigned char* p0 = lds + ly * LW + lx;
+            unsigned char* p1 = p0 + LW;
+            u32 c0 = p0[0], c1 = p0[1], c2 = p1[0], c3 = p1[1];
+            const int gx = gx0 + lx, gy = gy0 + ly;
+            se_block(c0, c1, c2, c3, (u32)gx, (u32)gy, f);
+            if (use_mods) {
+                int o;
+                if ((o = se_mod_override(mods, nmods, gx, gy)) >= 0) c0 = (u32)o;
+                if ((o = se_mod_override(mods, nmods, gx + 1, gy)) >= 0) c1 = (u32)o;
+                if ((o = se_mod_override(mods, nmods, gx, gy + 1)) >= 0) c2 = (u32)o;
+                if ((o = se_mod_override(mods, nmods, gx + 1, gy + 1)) >= 0) c3 = (u32)o;
+            }
+            if (interior) {
+                p0[0] = (unsigned char)c0; p0[1] = (unsigned char)c1;
+                p1[0] = (unsigned char)c2; p1[1] = (unsigned char)c3;
+            } else {
+                const bool inx0 = gx >= 0 && gx < W, inx1 = gx + 1 >= 0 && gx + 1 < W;
+                const bool iny0 = gy >= 0 && gy < H, iny1 = gy + 1 >= 0 && gy + 1 < H;
+                if (inx0 && iny0) p0[0] = (unsigned char)c0;
+                if (inx1 && iny0) p0[1] = (unsigned char)c1;
+                if (inx0 && iny1) p1[0] = (unsigned char)c2;
+                if (inx1 && iny1) p1[1] = (unsigned char)c3;
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < TW * TH; i += 256) {
+        const int ly = i / TW, lx = i - ly * TW;
+        const int gx = gx0 + HB + lx, gy = gy0 + HB + ly;
+        if (gx < W && gy < H) out[(size_t)gy * pitch + gx] = lds[(ly + HB) * LW + lx + HB];
+    }
+}
+
+#define SE_LDS_KERNEL(TW, TH, HB)                                                                              \
+    extern "C" __global__ __launch_bounds__(256) void se_step_lds_##TW##x##TH##_h##HB(                        \
+        const u32* __restrict__ in, u32* __restrict__ out, int W, int H, int pitch, int frame0, int nsub,     \
+        int tiles_x, const se_mod* __restrict__ mods, int nmods) {                                             \
+        __shared__ unsigned char lds[(TW + 2 * HB) * (TH + 2 * HB)];                                           \
+        se_lds_body<TW, TH, HB>(in, out, W, H, pitch, frame0, nsub, tiles_x, mods, nmods, lds);               \
+    }
+
+// ---------------------------------------------------------------------------
+// Seeded i.i.d. fill of the benchmark workloads: r = hi32(splitmix64((seed << 40)
+// ^ (y*W + x))); the first k < n-1 with r < thr[k] picks ids[k], else ids[n-1];
+// rows y >= H - stone_rows take stone_id.  Padding columns are left alone.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ u64 se_splitmix64(u64 z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+extern "C" __global__ __launch_bounds__(256) void se_fill_seeded(u32* __restrict__ g, int W, int H, int pitch,
+                                                                 u64 seed, const u32* __restrict__ ids,
+                                                                 const u32* __restrict__ thr, int n,
+                                                                 int stone_rows, u32 stone_id) {
+    const size_t total = (size_t)W * (size_t)H;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        const size_t y = i / (size_t)W, x = i - y * (size_t)W;
+        u32 v;
+        if ((long)y >= (long)H - stone_rows) {
+            v = stone_id;
+        } else {
+            const u32 r = (u32)(se_splitmix64((seed << 40) ^ (u64)i) >> 32);
+            v = ids[n - 1];
+            for (int k = 0; k < n - 1; k++)
+                if (r < thr[k]) { v = ids[k]; break; }
+        }
+        g[y * (size_t)pitch + x] = v;
+    }
+}

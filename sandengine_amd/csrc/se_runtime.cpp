@@ -1,0 +1,619 @@
+// Device runtime and C ABI (include/sandengine_hip.h).
+//
+// A se_world is the MI355X replacement for sandengine-core's Simulation
+// (sandengine-core/src/simulation.rs:97-253): the RGBA32F ping-pong textures
+// become two u32 grids in HBM with a 256-byte-aligned row pitch, the
+// 256-entry modification UBO becomes a small device array uploaded only for
+// steps that carry modifications, and the compute-shader dispatch becomes a
+// hiprtc-compiled kernel launch on the world's stream.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sandengine_hip.h"
+#include "se_hipgen.hpp"
+#include "se_lang.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHECK(expr)                                                                                 \
+    do {                                                                                               \
+        hipError_t _e = (expr);                                                                        \
+        if (_e != hipSuccess) return fail(SE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+char* dupstr(const std::string& s) {
+    char* p = (char*)std::malloc(s.size() + 1);
+    if (p) std::memcpy(p, s.c_str(), s.size() + 1);
+    return p;
+}
+
+const std::vector<se::LdsVariant>& lds_variants() {
+    static const std::vector<se::LdsVariant> v = {{256, 64, 8}, {128, 128, 8}, {256, 128, 8}, {256, 64, 16}};
+    return v;
+}
+
+// hiprtc options: gfx950, IEEE f32 with no contraction, and packed-f32 forms off
+const char* kJitOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+                          "-Wno-parentheses-equality", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+
+int jit_compile(const std::string& src, std::vector<char>& code) {
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "se_rules.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+        return fail(SE_ERR_JIT, "hiprtcCreateProgram failed");
+    hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof kJitOpts / sizeof kJitOpts[0]), kJitOpts);
+    if (r != HIPRTC_SUCCESS) {
+        size_t ls = 0;
+        hiprtcGetProgramLogSize(prog, &ls);
+        std::string log(ls + 1, '\0');
+        hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        return fail(SE_ERR_JIT, "hiprtc compile failed: " + log);
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.resize(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return SE_OK;
+}
+
+int parse_yaml(const char* yaml, size_t len, se::ParsingResult& pr, std::string* kind = nullptr) {
+    if (!yaml) return fail(SE_ERR_ARG, "null rule text");
+    try {
+        pr = se::parse_string(std::string(yaml, len));
+    } catch (const se::LangError& e) {
+        if (kind) *kind = e.kind;
+        return fail(SE_ERR_PARSE, e.what());
+    } catch (const std::exception& e) {
+        return fail(SE_ERR_PARSE, e.what());
+    }
+    return SE_OK;
+}
+
+int gen_source(const se::ParsingResult& pr, std::string& src) {
+    try {
+        src = se::hip_program_source(pr, lds_variants());
+    } catch (const se::LangError& e) {
+        return fail(SE_ERR_CODEGEN, e.what());
+    } catch (const std::exception& e) {
+        return fail(SE_ERR_CODEGEN, e.what());
+    }
+    return SE_OK;
+}
+
+// Launches `f` on a 1-D grid of 256-thread workgroups; each argument is passed
+// by address (kernelParams), so its C++ type must match the kernel parameter.
+template <typename... A>
+int launch(hipFunction_t f, unsigned grid, hipStream_t s, A... args) {
+    void* p[] = {(void*)&args...};
+    hipError_t e = hipModuleLaunchKernel(f, grid, 1, 1, 256, 1, 1, 0, s, p, nullptr);
+    if (e != hipSuccess) return fail(SE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return SE_OK;
+}
+
+uint64_t fnv1a(const std::string& s) {
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char c : s) h = (h ^ c) * 1099511628211ull;
+    return h;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ internal types
+struct se_module {
+    hipModule_t mod = nullptr;
+    hipFunction_t plain = nullptr, fill = nullptr, fill_seeded = nullptr;
+    std::vector<std::pair<se::LdsVariant, hipFunction_t>> lds;
+    int device = 0;
+    ~se_module() {
+        if (mod) {
+            int prev;
+            hipGetDevice(&prev);
+            hipSetDevice(device);
+            hipModuleUnload(mod);
+            hipSetDevice(prev);
+        }
+    }
+};
+
+struct se_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::map<uint64_t, std::pair<std::string, std::shared_ptr<se_module>>> cache;
+};
+
+struct se_ruleset {
+    se_ctx* ctx = nullptr;
+    se::ParsingResult pr;
+    std::shared_ptr<se_module> mod;
+    int nmat = 0;
+};
+
+struct se_world {
+    se_ctx* ctx = nullptr;
+    const se_ruleset* rs = nullptr;
+    std::shared_ptr<se_module> mod;
+    uint32_t W = 0, H = 0, pitch = 0;
+    uint32_t* buf[2] = {nullptr, nullptr};
+    int cur = 0;
+    int32_t frame = 0;
+    std::vector<se_modification> pending;
+    void* d_mods = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        hipGetDevice(&prev);
+        if (prev != dev) hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int now;
+        hipGetDevice(&now);
+        if (prev >= 0 && now != prev) hipSetDevice(prev);
+    }
+};
+
+int load_module(se_ctx* ctx, const std::string& src, std::shared_ptr<se_module>& out) {
+    uint64_t key = fnv1a(src);
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        auto it = ctx->cache.find(key);
+        if (it != ctx->cache.end() && it->second.first == src) {
+            out = it->second.second;
+            return SE_OK;
+        }
+    }
+    std::vector<char> code;
+    int rc = jit_compile(src, code);
+    if (rc) return rc;
+    DeviceGuard dg(ctx->device);
+    auto m = std::make_shared<se_module>();
+    m->device = ctx->device;
+    HIPCHECK(hipModuleLoadData(&m->mod, code.data()));
+    HIPCHECK(hipModuleGetFunction(&m->plain, m->mod, "se_step_plain"));
+    HIPCHECK(hipModuleGetFunction(&m->fill, m->mod, "se_fill"));
+    HIPCHECK(hipModuleGetFunction(&m->fill_seeded, m->mod, "se_fill_seeded"));
+    for (auto& v : lds_variants()) {
+        hipFunction_t f;
+        HIPCHECK(hipModuleGetFunction(&f, m->mod, se::lds_kernel_name(v).c_str()));
+        m->lds.push_back({v, f});
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->cache[key] = {src, m};
+    out = m;
+    return SE_OK;
+}
+
+void offset_of(int32_t frame, int& ox, int& oy) {
+    int m = frame % 4;  // operations.glsl:25-34
+    ox = (m == 1 || m == 3) ? 1 : 0;
+    oy = (m == 1 || m == 2) ? 1 : 0;
+}
+
+struct ModsArg {
+    void* ptr;
+    int n;
+};
+
+// Uploads the queued modifications (cut at the first size-0 entry, as the
+// kernel's loop stops there, falling_sand.glsl:139-141) and clears the queue.
+int take_mods(se_world* w, ModsArg& m) {
+    m.ptr = w->d_mods;
+    m.n = 0;
+    if (w->pending.empty()) return SE_OK;
+    int n = 0;
+    while (n < (int)w->pending.size() && n < SE_MAX_MODIFICATIONS && w->pending[n].mod_size != 0) n++;
+    if (n > 0)
+        HIPCHECK(hipMemcpyAsync(w->d_mods, w->pending.data(), sizeof(se_modification) * n, hipMemcpyHostToDevice,
+                                w->stream));
+    w->pending.clear();
+    m.n = n;
+    return SE_OK;
+}
+
+int launch_fill(se_world* w, uint32_t* g, uint32_t v) {
+    size_t n = (size_t)w->pitch * w->H;
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    return launch(w->mod->fill, blocks, w->stream, g, n, v);
+}
+
+// One Simulation::run with the plain kernel.
+int plain_step(se_world* w) {
+    w->frame += 1;
+    uint32_t* in = w->buf[w->cur];
+    uint32_t* out = w->buf[w->cur ^ 1];
+    if (w->frame == 1) {  // falling_sand.glsl:128-131
+        w->pending.clear();
+        int rc = launch_fill(w, out, 0u);
+        if (rc) return rc;
+        w->cur ^= 1;
+        return SE_OK;
+    }
+    ModsArg m;
+    int rc = take_mods(w, m);
+    if (rc) return rc;
+    int ox, oy;
+    offset_of(w->frame, ox, oy);
+    int strips = (int)((w->W + 255) / 256);
+    int brows = (int)((w->H + (uint32_t)oy + 1) / 2);
+    size_t waves = (size_t)strips * brows;
+    unsigned blocks = (unsigned)((waves + 3) / 4);
+    int rc2 = launch(w->mod->plain, blocks, w->stream, (const uint32_t*)in, out, (int)w->W, (int)w->H,
+                     (int)w->pitch, (int)w->frame, ox, oy, strips, brows, (const void*)m.ptr, m.n);
+    if (rc2) return rc2;
+    w->cur ^= 1;
+    return SE_OK;
+}
+
+const std::pair<se::LdsVariant, hipFunction_t>* pick_lds(se_world* w, uint32_t sub) {
+    const std::pair<se::LdsVariant, hipFunction_t>* best = nullptr;
+    for (auto& p : w->mod->lds) {
+        if (sub == 0) {  // default: 256x64 tile, 8 sub-steps
+            if (p.first.tw == 256 && p.first.th == 64 && p.first.hb == 8) return &p;
+            continue;
+        }
+        if ((uint32_t)p.first.hb < sub) continue;
+        if (!best || p.first.hb < best->first.hb ||
+            (p.first.hb == best->first.hb && p.first.tw * p.first.th > best->first.tw * best->first.th))
+            best = &p;
+    }
+    return best;
+}
+
+// Advances `n` <= hb frames with one LDS launch (mods, if any, at its first).
+int lds_launch(se_world* w, const std::pair<se::LdsVariant, hipFunction_t>& v, int n) {
+    ModsArg m;
+    int rc = take_mods(w, m);
+    if (rc) return rc;
+    const se::LdsVariant& L = v.first;
+    int tiles_x = (int)((w->W + L.tw - 1) / L.tw), tiles_y = (int)((w->H + L.th - 1) / L.th);
+    rc = launch(v.second, (unsigned)(tiles_x * tiles_y), w->stream, (const uint32_t*)w->buf[w->cur],
+                w->buf[w->cur ^ 1], (int)w->W, (int)w->H, (int)w->pitch, (int)(w->frame + 1), n, tiles_x,
+                (const void*)m.ptr, m.n);
+    if (rc) return rc;
+    w->frame += n;
+    w->cur ^= 1;
+    return SE_OK;
+}
+
+int blocked_steps(se_world* w, uint32_t nsteps, uint32_t sub) {
+    auto* v = pick_lds(w, sub);
+    if (!v) return fail(SE_ERR_ARG, "no LDS variant supports " + std::to_string(sub) + " sub-steps");
+    uint32_t k = sub ? sub : (uint32_t)v->first.hb;
+    uint32_t done = 0;
+    while (done < nsteps) {
+        if (w->frame + 1 == 1) {  // the clearing frame stays a step of its own
+            int rc = plain_step(w);
+            if (rc) return rc;
+            done++;
+            continue;
+        }
+        uint32_t n = std::min<uint32_t>(k, nsteps - done);
+        int rc = lds_launch(w, *v, (int)n);
+        if (rc) return rc;
+        done += n;
+    }
+    return SE_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+const char* se_last_error(void) { return g_err.c_str(); }
+const char* se_version(void) { return "sandengine-hip 0.1 (gfx950)"; }
+void se_free(void* p) { std::free(p); }
+
+int se_parse_ir_json(const char* yaml, size_t len, char** json_out, char** kind_out) {
+    if (json_out) *json_out = nullptr;
+    if (kind_out) *kind_out = nullptr;
+    se::ParsingResult pr;
+    std::string kind;
+    int rc = parse_yaml(yaml, len, pr, &kind);
+    if (rc) {
+        if (kind_out) *kind_out = dupstr(kind);
+        return rc;
+    }
+    if (json_out) *json_out = dupstr(se::ir_json(pr));
+    return SE_OK;
+}
+
+int se_emit_glsl(const char* yaml, size_t len, char** materials_out, char** rules_out) {
+    se::ParsingResult pr;
+    int rc = parse_yaml(yaml, len, pr);
+    if (rc) return rc;
+    auto g = se::create_glsl(pr);
+    if (materials_out) *materials_out = dupstr(g.first);
+    if (rules_out) *rules_out = dupstr(g.second);
+    return SE_OK;
+}
+
+int se_emit_hip(const char* yaml, size_t len, char** src_out) {
+    se::ParsingResult pr;
+    int rc = parse_yaml(yaml, len, pr);
+    if (rc) return rc;
+    std::string src;
+    rc = gen_source(pr, src);
+    if (rc) return rc;
+    if (src_out) *src_out = dupstr(src);
+    return SE_OK;
+}
+
+int se_jit_check(const char* yaml, size_t len, size_t* code_bytes) {
+    se::ParsingResult pr;
+    int rc = parse_yaml(yaml, len, pr);
+    if (rc) return rc;
+    std::string src;
+    rc = gen_source(pr, src);
+    if (rc) return rc;
+    std::vector<char> code;
+    rc = jit_compile(src, code);
+    if (rc) return rc;
+    if (code_bytes) *code_bytes = code.size();
+    return SE_OK;
+}
+
+int se_ctx_create(int device, se_ctx** out) {
+    if (!out) return fail(SE_ERR_ARG, "null out");
+    *out = nullptr;
+    int n = 0;
+    HIPCHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(SE_ERR_ARG, "device " + std::to_string(device) + " not present");
+    DeviceGuard dg(device);
+    auto* c = new se_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(SE_ERR_HIP, "hipStreamCreate failed");
+    }
+    *out = c;
+    return SE_OK;
+}
+
+void se_ctx_destroy(se_ctx* ctx) {
+    if (!ctx) return;
+    {
+        DeviceGuard dg(ctx->device);
+        hipStreamSynchronize(ctx->stream);
+        hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+int se_ctx_mem_info(se_ctx* ctx, size_t* total, size_t* free_bytes) {
+    if (!ctx) return fail(SE_ERR_ARG, "null ctx");
+    DeviceGuard dg(ctx->device);
+    size_t f = 0, t = 0;
+    HIPCHECK(hipMemGetInfo(&f, &t));
+    if (total) *total = t;
+    if (free_bytes) *free_bytes = f;
+    return SE_OK;
+}
+
+int se_ruleset_from_yaml(se_ctx* ctx, const char* yaml, size_t len, se_ruleset** out) {
+    if (!ctx || !out) return fail(SE_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto rs = std::make_unique<se_ruleset>();
+    int rc = parse_yaml(yaml, len, rs->pr);
+    if (rc) return rc;
+    std::string src;
+    rc = gen_source(rs->pr, src);
+    if (rc) return rc;
+    rc = load_module(ctx, src, rs->mod);
+    if (rc) return rc;
+    rs->ctx = ctx;
+    rs->nmat = (int)rs->pr.materials.size();
+    *out = rs.release();
+    return SE_OK;
+}
+
+void se_ruleset_destroy(se_ruleset* rs) { delete rs; }
+
+int se_ruleset_material_id(const se_ruleset* rs, const char* name, int32_t* id) {
+    if (!rs || !name || !id) return fail(SE_ERR_ARG, "null argument");
+    for (auto& m : rs->pr.materials)
+        if (m.name == name) {
+            *id = (int32_t)m.id;
+            return SE_OK;
+        }
+    return fail(SE_ERR_ARG, std::string("no material named '") + name + "'");
+}
+
+int se_ruleset_material_count(const se_ruleset* rs, int32_t* n) {
+    if (!rs || !n) return fail(SE_ERR_ARG, "null argument");
+    *n = rs->nmat;
+    return SE_OK;
+}
+
+int se_world_create(se_ctx* ctx, const se_ruleset* rs, uint32_t w, uint32_t h, se_world** out) {
+    if (!ctx || !rs || !out) return fail(SE_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (w == 0 || h == 0 || w > (1u << 20) || h > (1u << 20))
+        return fail(SE_ERR_ARG, "world size must be in [1, 2^20] per side");
+    if (rs->ctx != ctx) return fail(SE_ERR_ARG, "rule set belongs to another context");
+    DeviceGuard dg(ctx->device);
+    auto* W = new se_world();
+    W->ctx = ctx;
+    W->rs = rs;
+    W->mod = rs->mod;
+    W->W = w;
+    W->H = h;
+    W->pitch = (w + 63u) & ~63u;
+    W->stream = ctx->stream;
+    size_t bytes = (size_t)W->pitch * h * sizeof(uint32_t);
+    if (hipMalloc(&W->buf[0], bytes) != hipSuccess || hipMalloc(&W->buf[1], bytes) != hipSuccess ||
+        hipMalloc(&W->d_mods, sizeof(se_modification) * SE_MAX_MODIFICATIONS) != hipSuccess) {
+        hipFree(W->buf[0]);
+        hipFree(W->buf[1]);
+        hipFree(W->d_mods);
+        delete W;
+        hipGetLastError();
+        return fail(SE_ERR_OOM, "hipMalloc of " + std::to_string(2 * bytes) + " bytes failed");
+    }
+    int rc = launch_fill(W, W->buf[0], 0u);
+    if (!rc) rc = launch_fill(W, W->buf[1], 0u);
+    if (rc) {
+        se_world_destroy(W);
+        return rc;
+    }
+    *out = W;
+    return SE_OK;
+}
+
+void se_world_destroy(se_world* w) {
+    if (!w) return;
+    {
+        DeviceGuard dg(w->ctx->device);
+        hipStreamSynchronize(w->stream);
+        hipFree(w->buf[0]);
+        hipFree(w->buf[1]);
+        hipFree(w->d_mods);
+    }
+    delete w;
+}
+
+int se_world_upload(se_world* w, const uint32_t* cells, size_t n) {
+    if (!w || !cells) return fail(SE_ERR_ARG, "null argument");
+    if (n != (size_t)w->W * w->H) return fail(SE_ERR_ARG, "upload size mismatch");
+    DeviceGuard dg(w->ctx->device);
+    HIPCHECK(hipMemcpy2DAsync(w->buf[w->cur], (size_t)w->pitch * 4, cells, (size_t)w->W * 4, (size_t)w->W * 4, w->H,
+                              hipMemcpyHostToDevice, w->stream));
+    HIPCHECK(hipStreamSynchronize(w->stream));
+    return SE_OK;
+}
+
+int se_world_download(se_world* w, uint32_t* cells, size_t n) {
+    if (!w || !cells) return fail(SE_ERR_ARG, "null argument");
+    if (n != (size_t)w->W * w->H) return fail(SE_ERR_ARG, "download size mismatch");
+    DeviceGuard dg(w->ctx->device);
+    HIPCHECK(hipMemcpy2DAsync(cells, (size_t)w->W * 4, w->buf[w->cur], (size_t)w->pitch * 4, (size_t)w->W * 4, w->H,
+                              hipMemcpyDeviceToHost, w->stream));
+    HIPCHECK(hipStreamSynchronize(w->stream));
+    return SE_OK;
+}
+
+int se_world_push_modification(se_world* w, const se_modification* m) {
+    if (!w || !m) return fail(SE_ERR_ARG, "null argument");
+    if (w->pending.size() >= SE_MAX_MODIFICATIONS)
+        return fail(SE_ERR_FULL, "more than 256 modifications queued for one step");
+    w->pending.push_back(*m);
+    return SE_OK;
+}
+
+int se_world_step(se_world* w, uint32_t nsteps) {
+    if (!w) return fail(SE_ERR_ARG, "null world");
+    if ((int64_t)w->frame + nsteps > INT32_MAX) return fail(SE_ERR_ARG, "frame counter would overflow");
+    DeviceGuard dg(w->ctx->device);
+    for (uint32_t s = 0; s < nsteps; s++) {
+        int rc = plain_step(w);
+        if (rc) return rc;
+    }
+    return SE_OK;
+}
+
+int se_world_step_blocked(se_world* w, uint32_t nsteps, uint32_t sub) {
+    if (!w) return fail(SE_ERR_ARG, "null world");
+    if ((int64_t)w->frame + nsteps > INT32_MAX) return fail(SE_ERR_ARG, "frame counter would overflow");
+    DeviceGuard dg(w->ctx->device);
+    return blocked_steps(w, nsteps, sub);
+}
+
+int se_world_get_frame(const se_world* w, int32_t* frame) {
+    if (!w || !frame) return fail(SE_ERR_ARG, "null argument");
+    *frame = w->frame;
+    return SE_OK;
+}
+
+int se_world_set_frame(se_world* w, int32_t frame) {
+    if (!w) return fail(SE_ERR_ARG, "null world");
+    if (frame < 0) return fail(SE_ERR_ARG, "negative frame");
+    w->frame = frame;
+    return SE_OK;
+}
+
+int se_world_sync(se_world* w) {
+    if (!w) return fail(SE_ERR_ARG, "null world");
+    DeviceGuard dg(w->ctx->device);
+    HIPCHECK(hipStreamSynchronize(w->stream));
+    return SE_OK;
+}
+
+int se_world_device_ptr(se_world* w, void** ptr, uint32_t* pitch) {
+    if (!w) return fail(SE_ERR_ARG, "null world");
+    if (ptr) *ptr = w->buf[w->cur];
+    if (pitch) *pitch = w->pitch;
+    return SE_OK;
+}
+
+int se_world_fill_seeded(se_world* w, uint64_t seed, const uint32_t* ids, const uint32_t* thresholds, int n,
+                         uint32_t stone_rows, uint32_t stone_id) {
+    if (!w || !ids || !thresholds || n < 1 || n > 64) return fail(SE_ERR_ARG, "bad fill arguments");
+    DeviceGuard dg(w->ctx->device);
+    void* d = nullptr;
+    HIPCHECK(hipMalloc(&d, 2 * 64 * sizeof(uint32_t)));
+    uint32_t host[128] = {0};
+    std::memcpy(host, ids, n * 4);
+    std::memcpy(host + 64, thresholds, n * 4);
+    hipError_t e = hipMemcpy(d, host, sizeof host, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(d);
+        return fail(SE_ERR_HIP, hipGetErrorString(e));
+    }
+    size_t cells = (size_t)w->W * w->H;
+    unsigned blocks = (unsigned)std::min<size_t>((cells + 255) / 256, 1u << 16);
+    int rc = launch(w->mod->fill_seeded, blocks, w->stream, w->buf[w->cur], (int)w->W, (int)w->H, (int)w->pitch,
+                    (unsigned long long)seed, (const uint32_t*)d, (const uint32_t*)d + 64, n, (int)stone_rows,
+                    stone_id);
+    hipError_t e2 = hipStreamSynchronize(w->stream);
+    hipFree(d);
+    if (rc) return rc;
+    if (e2 != hipSuccess) return fail(SE_ERR_HIP, std::string("fill sync: ") + hipGetErrorString(e2));
+    return SE_OK;
+}
+
+int se_world_step_timed(se_world* w, uint32_t nsteps, int mode, uint32_t sub, float* gpu_ms) {
+    if (!w || !gpu_ms) return fail(SE_ERR_ARG, "null argument");
+    DeviceGuard dg(w->ctx->device);
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    hipEventRecord(a, w->stream);
+    int rc = mode == 0 ? se_world_step(w, nsteps) : blocked_steps(w, nsteps, sub);
+    hipEventRecord(b, w->stream);
+    hipError_t e = hipEventSynchronize(b);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(SE_ERR_HIP, hipGetErrorString(e));
+    *gpu_ms = ms;
+    return SE_OK;
+}
+
+}  // extern "C"

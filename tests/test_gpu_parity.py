@@ -1,0 +1,176 @@
+"""GPU parity: the HIP kernels against the llvmpipe golden fixtures and the oracle.
+
+Every comparison is bit-exact (integer cell ids).  Sizes the oracle finishes in
+seconds are compared cell for cell; at the 16384^2 benchmark size the checks are
+size-independent properties (plain == LDS-blocked, per-material counts
+conserved by swap-only rules, WALL/NULL cells fixed).
+"""
+import numpy as np
+import pytest
+
+import sandengine_amd as S
+from oracle import lang_ref as L
+from oracle import oracle as O
+from tests.golden import cases as C
+from tests.golden import load
+
+pytestmark = pytest.mark.gpu
+
+ALL_CASES = load.cases()
+_ctx = None
+_rulesets = {}
+
+
+def ctx():
+    global _ctx
+    if _ctx is None:
+        _ctx = S.Context(0)
+    return _ctx
+
+
+def ruleset(name):
+    text = load.ruleset_text(name)
+    if text is None:
+        pytest.skip("reference rule file not present on this machine")
+    if name not in _rulesets:
+        _rulesets[name] = S.RuleSet(ctx(), text)
+    return _rulesets[name]
+
+
+def run_protocol(rs, g0, first_frame, steps, mods_at, mode, sub=0, checkpoints=()):
+    """create (frame 0) -> one clearing step -> upload -> frames first_frame.."""
+    H, W = g0.shape
+    w = S.World(ctx(), rs, W, H)
+    w.step(1)
+    w.upload(g0)
+    w.frame = first_frame - 1
+    stops = sorted(set([k for k in mods_at if k < steps] + [c for c in checkpoints if c < steps] + [steps]))
+    snaps = {}
+    s = 0
+    adv = w.step if mode == "plain" else (lambda n: w.step_blocked(n, sub))
+    for stop in stops:
+        if s in mods_at:
+            for m in mods_at[s]:
+                w.push_modification(*m)
+        if stop > s:
+            adv(stop - s)
+            s = stop
+        if s in checkpoints:
+            snaps[s] = w.download()
+    out = w.download()
+    frame = w.frame
+    w.close()
+    return out, snaps, frame
+
+
+@pytest.mark.parametrize("mode,sub", [("plain", 0), ("blocked", 0), ("blocked", 3)])
+@pytest.mark.parametrize("case", ALL_CASES, ids=[c["name"] for c in ALL_CASES])
+def test_golden(case, mode, sub):
+    print(f"\n[gpu] golden {case['name']} {mode} sub={sub}", flush=True)
+    rs = ruleset(case["ruleset"])
+    g0 = C.make_input(case)
+    cps = tuple(case.get("checkpoints", ()))
+    out, snaps, frame = run_protocol(rs, g0, case.get("first_frame", 2), case["steps"], load.mods_at(case),
+                                     mode, sub, cps)
+    assert frame == case.get("first_frame", 2) - 1 + case["steps"]
+    for k, v in snaps.items():
+        np.testing.assert_array_equal(v, load.grid(f"{case['name']}@{k}"))
+    np.testing.assert_array_equal(out, load.grid(case["name"]))
+
+
+RANDOM_SHAPES = [(1, 1), (2, 1), (3, 7), (255, 3), (256, 5), (257, 9), (300, 301), (513, 64), (700, 33),
+                 (1030, 17), (64, 1000)]
+
+
+@pytest.mark.parametrize("rsname", ["four", "twelve", "features"])
+@pytest.mark.parametrize("shape", RANDOM_SHAPES, ids=[f"{w}x{h}" for w, h in RANDOM_SHAPES])
+def test_random_vs_oracle(rsname, shape):
+    W, H = shape
+    print(f"\n[gpu] random {rsname} {W}x{H}", flush=True)
+    rs = ruleset(rsname)
+    blob = O.compile_ruleset(L.parse_string(load.ruleset_text(rsname)))
+    rng = np.random.RandomState(W * 7919 + H)
+    case = dict(ruleset=rsname, W=W, H=H, seed=int(rng.randint(0, 10 ** 6)), specials=0.02)
+    g0 = C.make_input(case)
+    steps = 37
+    mods_at = {0: C._rand_mods(rng, W, H, C.NMAT[rsname], 3), 11: C._rand_mods(rng, W, H, C.NMAT[rsname], 2),
+               20: C._rand_mods(rng, W, H, C.NMAT[rsname], 4)}
+    first = int(rng.randint(2, 50))
+    want = g0
+    for s in range(steps):
+        want = O.step(blob, want, first + s, mods_at.get(s))
+    for mode, sub in [("plain", 0), ("blocked", 0), ("blocked", 5)]:
+        got, _, _ = run_protocol(rs, g0, first, steps, mods_at, mode, sub)
+        np.testing.assert_array_equal(got, want, err_msg=f"{mode} sub={sub}")
+
+
+def test_clear_frame_and_api_errors():
+    rs = ruleset("four")
+    w = S.World(ctx(), rs, 40, 30)
+    w.upload(np.full((30, 40), 3, np.uint32))
+    w.push_modification(20, 15, 0, 5, 5)
+    w.step(1)  # frame 1 clears, modifications included
+    assert w.frame == 1
+    assert not w.download().any()
+    with pytest.raises(S.SandengineError):
+        w.frame = -3
+    for i in range(S.MAX_MODIFICATIONS):
+        w.push_modification(0, 0, 0, 1, 3)
+    with pytest.raises(S.SandengineError):
+        w.push_modification(0, 0, 0, 1, 3)
+    with pytest.raises(S.SandengineError):
+        w.upload(np.zeros((3, 3), np.uint32))
+    w.close()
+
+
+def test_simulation_api_matches_oracle():
+    text = S.bundled_ruleset("twelve_materials")
+    blob = O.compile_ruleset(L.parse_string(text))
+    sim = S.Simulation((96, 64), text, ctx=ctx())
+    sim.run()  # frame 1 clears
+    g0 = O.fill(4, 96, 64, [0] + list(range(3, 15)), [40] + [5] * 12)
+    sim.upload(g0)
+    want = g0
+    for f in range(2, 40):
+        mods = [[48, 10, 0, 8, 7]] if f == 10 else None
+        if mods:
+            sim.modifications.append(S.SimModification((48, 10), S.MODSHAPE_CIRCLE, 8, 7))
+        sim.run()
+        want = O.step(blob, want, f, mods)
+    assert sim.params.frame == 39
+    np.testing.assert_array_equal(sim.grid, want)
+
+
+def _config1(w):
+    w.fill_seeded(1, [0, 3, 4], [60, 25, 15], stone_rows=(w.H * 3) // 10, stone_id=5)
+
+
+def test_device_fill_matches_generator():
+    rs = ruleset("four")
+    w = S.World(ctx(), rs, 333, 211)
+    _config1(w)
+    want = O.fill(1, 333, 211, [0, 3, 4], [60, 25, 15], (211 * 3) // 10, 5)
+    np.testing.assert_array_equal(w.download(), want)
+    w.close()
+
+
+@pytest.mark.parametrize("side,steps", [(4096, 64), (16384, 24)])
+def test_large_plain_equals_blocked_and_conserves(side, steps):
+    print(f"\n[gpu] large {side}^2 {steps} steps plain vs blocked", flush=True)
+    rs = ruleset("four")
+    a = S.World(ctx(), rs, side, side)
+    b = S.World(ctx(), rs, side, side)
+    for w in (a, b):
+        w.frame = 1
+        _config1(w)
+    g0 = a.download()
+    counts = np.bincount(g0.ravel(), minlength=6)
+    del g0
+    a.step(steps)
+    b.step_blocked(steps)
+    ga, gb = a.download(), b.download()
+    assert a.frame == b.frame == 1 + steps
+    np.testing.assert_array_equal(ga, gb)
+    np.testing.assert_array_equal(np.bincount(ga.ravel(), minlength=6), counts)
+    a.close()
+    b.close()
