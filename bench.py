@@ -138,6 +138,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--latency-samples", type=int, default=50)
+    ap.add_argument("--copy-reps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU). "
@@ -152,6 +153,13 @@ def main():
     log(f"ruleset {rsname}: JIT {time.perf_counter() - t0:.2f}s")
     cells = side * side
     results, worlds = {}, {}
+    # HBM reference: a streaming copy of the same grid (reads and writes 8 B per cell)
+    wc = make_world(S, ctx, rs, args.workload)
+    wc.copy_timed(4)
+    copy_ms = wc.copy_timed(args.copy_reps) / args.copy_reps
+    wc.close()
+    copy_gbs = 8.0 * cells / (copy_ms / 1e3) / 1e9
+    log(f"copy reference: {copy_ms:.3f} ms per 2x{4 * cells / 2**30:.2f} GiB -> {copy_gbs:.0f} GB/s")
     for mode in args.paths.split(","):
         w = make_world(S, ctx, rs, args.workload)
         wall, gpu_ms = time_path(S, w, mode, args.steps, args.warmup, args.sub)
@@ -161,6 +169,8 @@ def main():
                  ms_per_step=wall * 1e3 / args.steps, avg_launch_ms=avg_launch_ms, launches=launches,
                  steps_per_launch=1 if mode == "plain" else args.sub)
         r["achieved_gbs"] = 8.0 * cells / (avg_launch_ms / 1e3) / 1e9
+        r["roofline_frac"] = r["achieved_gbs"] / HBM_PEAK_GBS
+        r["frac_of_copy"] = r["achieved_gbs"] / copy_gbs
         p50, p99 = latencies(w, mode, args.sub, args.latency_samples)
         r["step_latency_ms"] = {"p50": p50, "p99": p99}
         log(f"{mode}: {r['gcups']:.1f} Gcell-upd/s wall, {r['gcups_gpu']:.1f} by events, "
@@ -184,7 +194,7 @@ def main():
         w.close()
     best = max(results, key=lambda m: results[m]["gcups"])
     R = results[best]
-    kernel = "se_step_plain" if best == "plain" else "se_step_lds"
+    kernel = "se_step_plain_r1_nt1" if best == "plain" else "se_step_lds_128x128_h8"
     traffic = load_traffic(kernel)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload)
     out = {
@@ -206,6 +216,9 @@ def main():
                      "unit": "GB/s", "frac": R["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": 8 * cells, "avg_launch_ms": R["avg_launch_ms"]},
         "cpu_baseline": cpu,
+        "copy_reference": {"gbs": copy_gbs, "ms": copy_ms, "frac_of_peak": copy_gbs / HBM_PEAK_GBS},
+        "speedup_blocked_vs_plain": (results["blocked"]["gcups"] / results["plain"]["gcups"]
+                                     if "blocked" in results and "plain" in results else None),
         "paths": results,
         "parity": check,
     }

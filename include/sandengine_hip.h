@@ -113,6 +113,13 @@ int se_world_step(se_world* w, uint32_t nsteps);
 /* Same result as se_world_step, advanced by the LDS temporal-blocked kernel in
  * launches of up to `sub` sub-steps (0 = autotuned choice). */
 int se_world_step_blocked(se_world* w, uint32_t nsteps, uint32_t sub);
+/* Kernel variant selection (no reference counterpart; the autotuner drives it):
+ * plain_variant indexes the plain-kernel instantiations (rows per wavefront x
+ * store kind), lds_variant the LDS-tile instantiations (-1: default).
+ * se_variant_info(kind 0 plain: a=rows, b=non-temporal; kind 1 lds: a=tile w,
+ * b=tile h, c=max sub-steps). */
+int se_world_set_variant(se_world* w, int32_t plain_variant, int32_t lds_variant);
+int se_variant_info(const se_world* w, int32_t kind, int32_t index, int32_t* a, int32_t* b, int32_t* c);
 /* Params.frame (simulation.rs:78, 201). Negative frames are rejected. */
 int se_world_get_frame(const se_world* w, int32_t* frame);
 int se_world_set_frame(se_world* w, int32_t frame);
@@ -127,6 +134,10 @@ int se_world_fill_seeded(se_world* w, uint64_t seed, const uint32_t* ids, const 
 /* Runs nsteps (mode 0 plain, 1 blocked with `sub`) between two HIP events on
  * the world's stream; *gpu_ms receives their elapsed time. */
 int se_world_step_timed(se_world* w, uint32_t nsteps, int mode, uint32_t sub, float* gpu_ms);
+/* `reps` streaming copies of the grid between the world's two buffers (the
+ * same bytes a plain step reads and writes, no rules): the measured HBM
+ * reference for the plain kernel.  Leaves the grid contents unchanged. */
+int se_world_copy_timed(se_world* w, uint32_t reps, float* gpu_ms);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,9 @@ def lib():
         L.se_world_fill_seeded.argtypes = [c_p, ctypes.c_uint64, ctypes.POINTER(c_u32),
                                            ctypes.POINTER(c_u32), ctypes.c_int, c_u32, c_u32]
         L.se_world_step_timed.argtypes = [c_p, c_u32, ctypes.c_int, c_u32, ctypes.POINTER(ctypes.c_float)]
+        L.se_world_set_variant.argtypes = [c_p, c_i32, c_i32]
+        L.se_world_copy_timed.argtypes = [c_p, c_u32, ctypes.POINTER(ctypes.c_float)]
+        L.se_variant_info.argtypes = [c_p, c_i32, c_i32] + [ctypes.POINTER(c_i32)] * 3
         _lib = L
     return _lib
 
@@ -321,6 +324,20 @@ class World:
         ms = ctypes.c_float()
         _check(lib().se_world_step_timed(self._p, n, mode, sub, ctypes.byref(ms)))
         return ms.value
+
+    def copy_timed(self, reps: int) -> float:
+        """GPU ms of `reps` streaming copies of the grid (HBM reference)."""
+        ms = ctypes.c_float()
+        _check(lib().se_world_copy_timed(self._p, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def set_variant(self, plain: int = 3, lds: int = -1):
+        _check(lib().se_world_set_variant(self._p, plain, lds))
+
+    def variant_info(self, kind: int, index: int) -> Tuple[int, int, int]:
+        a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().se_variant_info(self._p, kind, index, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
 
     @property
     def frame(self) -> int:

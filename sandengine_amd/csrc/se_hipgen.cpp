@@ -727,19 +727,33 @@ std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info) {
     o << "};\n    return t[c];\n}\n";
     o << "__device__ __forceinline__ unsigned int se_lowbias(unsigned int h) {\n"
          "    h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;\n    return h;\n}\n";
-    o << "#define SE_GSWAP(a, b) do { const bool _ok = !se_fixed(a) && !se_fixed(b); const unsigned int _t = "
-         "_ok ? (b) : (a); (b) = _ok ? (a) : (b); (a) = _t; } while (0)\n";
+    // guarded swap on register values (copies first: a conditional over lvalues
+    // would select between addresses and keep the cells out of registers)
+    o << "#define SE_GSWAP(a, b) do { const unsigned int _a = (a), _b = (b); "
+         "const bool _ok = !se_fixed(_a) && !se_fixed(_b); (a) = _ok ? _b : _a; (b) = _ok ? _a : _b; } while (0)\n";
     // se_block: simulate() for one block (falling_sand.glsl:61-118)
     int64_t tmirror = rand_threshold(0.5f, false);
     static const unsigned mult[4] = {213u, 2131u, 21313u, 213132u};
-    o << "__device__ __forceinline__ void se_block(unsigned int& c0, unsigned int& c1, unsigned int& c2, "
-         "unsigned int& c3, unsigned int ox, unsigned int oy, unsigned int frame) {\n";
+    // The rule chain works on local copies, written back once at the end: stores
+    // through the reference parameters inside the branches would be merged into
+    // stores through pointer phis, which keeps the cells in scratch memory.
+    o << "__device__ __forceinline__ void se_block(unsigned int& r0, unsigned int& r1, unsigned int& r2, "
+         "unsigned int& r3, unsigned int ox, unsigned int oy, unsigned int frame) {\n";
+    o << "    unsigned int c0 = r0, c1 = r1, c2 = r2, c3 = r3;\n";
     o << "    if ((c0 | c1 | c2 | c3) == 0u) return;\n";
-    o << "    const unsigned int x = ox * 461u + oy * 2131u + frame * 4541161u;\n";
-    o << "    const unsigned int h0 = se_lowbias(x * 213u);\n";
-    for (int l = 1; l < 4; l++)
-        if (em.lanes_used.count(l))
-            o << "    const unsigned int h" << l << " = se_lowbias(x * " << mult[l] << "u);\n";
+    // x = ox*461 + oy*2131 + frame*2131*2131 (math.glsl:74-79) and lane k hashes
+    // x*m_k; the products are distributed so that the per-block terms use 24-bit
+    // multiplies where the constant fits (|ox|, |oy| < 2^23; -1 stays -1).
+    for (int l = 0; l < 4; l++) {
+        if (l > 0 && !em.lanes_used.count(l)) continue;
+        uint32_t A = 461u * mult[l], B = 2131u * mult[l], Cf = 4541161u * mult[l];
+        auto term = [](uint32_t k, const char* v) -> std::string {
+            if (k < (1u << 23)) return "(unsigned int)__mul24((int)" + std::string(v) + ", " + std::to_string(k) + ")";
+            return std::string(v) + " * " + std::to_string(k) + "u";
+        };
+        o << "    const unsigned int h" << l << " = se_lowbias(" << term(A, "ox") << " + " << term(B, "oy")
+          << " + frame * " << Cf << "u);\n";
+    }
     o << "    const bool mirror = h0 <= " << hex_u32((uint64_t)tmirror) << ";\n";
     o << "    if (mirror) { SE_GSWAP(c0, c1); SE_GSWAP(c2, c3); }\n";
     o << mirrored;
@@ -749,7 +763,7 @@ std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info) {
         std::string ln;
         while (std::getline(is, ln)) o << "    " << ln << "\n";
     }
-    o << "    }\n}\n";
+    o << "    }\n    r0 = c0; r1 = c1; r2 = c2; r3 = c3;\n}\n";
     if (info) {
         info->nmat = n;
         info->nrules = nrules;

@@ -112,87 +112,138 @@ __device__ __forceinline__ void se_apply_mods_quad(uint4& q, const se_mod* __res
     if ((o = se_mod_override(mods, nmods, x0 + 3, y)) >= 0) q.w = (u32)o;
 }
 
-extern "C" __global__ __launch_bounds__(256) void se_step_plain(
-    const u32* __restrict__ in, u32* __restrict__ out, int W, int H, int pitch, int frame, int offx, int offy,
-    int strips, int brows, const se_mod* __restrict__ mods, int nmods) {
+typedef unsigned int se_u32x4 __attribute__((ext_vector_type(4)));
+
+template <bool NT>
+__device__ __forceinline__ void se_store_quad(u32* p, const uint4& q) {
+    if (NT) {
+        se_u32x4 v = {q.x, q.y, q.z, q.w};
+        __builtin_nontemporal_store(v, reinterpret_cast<se_u32x4*>(p));
+    } else {
+        *reinterpret_cast<uint4*>(p) = q;
+    }
+}
+
+// One wavefront handles R consecutive block rows of one strip: all 2R row loads
+// are issued before any block is evaluated, to keep more bytes in flight.
+template <int R, bool NT>
+__device__ __forceinline__ void se_plain_body(const u32* __restrict__ in, u32* __restrict__ out, int W, int H,
+                                              int pitch, int frame, int offx, int offy, int strips, int brows,
+                                              const se_mod* __restrict__ mods, int nmods) {
     const int lane = threadIdx.x & 63;
     const int wave = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     const int strip = wave % strips;
-    const int brow = wave / strips;
-    if (brow >= brows) return;  // whole wavefront leaves together
-    const int ya = 2 * brow - offy, yb = ya + 1;
+    const int brow0 = (wave / strips) * R;
+    if (brow0 >= brows) return;  // whole wavefront leaves together
     const int base = strip * 256;
     const int x0 = base + 4 * lane;
-    const u32* rowa = (ya >= 0 && ya < H) ? in + (size_t)ya * pitch : nullptr;
-    const u32* rowb = (yb < H) ? in + (size_t)yb * pitch : nullptr;
-    uint4 a = se_load_quad(rowa, x0, W);
-    uint4 b = se_load_quad(rowb, x0, W);
     const u32 fr = (u32)frame;
-    if (offx == 0) {
-        se_block(a.x, a.y, b.x, b.y, (u32)x0, (u32)ya, fr);
-        se_block(a.z, a.w, b.z, b.w, (u32)(x0 + 2), (u32)ya, fr);
-    } else {
-        // cell x0+4 of both rows: lane+1's first cell; lane 63 loads it
-        u32 ea = __shfl_down(a.x, 1, 64), eb = __shfl_down(b.x, 1, 64);
-        if (lane == 63) {
-            const int xe = base + 256;
-            ea = rowa ? se_cell_at(rowa, xe, W) : SE_WALLID;
-            eb = rowb ? se_cell_at(rowb, xe, W) : SE_WALLID;
-        }
-        u32 edge_a = 0, edge_b = 0;
-        if (strip == 0 && lane == 0) {  // block at x = -1: left column is WALL
-            u32 w0 = SE_WALLID, w2 = SE_WALLID;
-            edge_a = a.x;
-            edge_b = b.x;
-            se_block(w0, edge_a, w2, edge_b, 0xFFFFFFFFu, (u32)ya, fr);
-        }
-        se_block(a.y, a.z, b.y, b.z, (u32)(x0 + 1), (u32)ya, fr);
-        se_block(a.w, ea, b.w, eb, (u32)(x0 + 3), (u32)ya, fr);
-        // new value of cell x0 comes from lane-1's second block
-        u32 pa = __shfl_up(ea, 1, 64), pb = __shfl_up(eb, 1, 64);
-        if (lane == 0) {
-            pa = edge_a;
-            pb = edge_b;
-        }
-        if (lane == 63 && base + 256 < W) {  // the cell past the strip
-            const int xe = base + 256;
-            u32 va = ea, vb = eb;
-            if (nmods > 0) {
-                int o;
-                if ((o = se_mod_override(mods, nmods, xe, ya)) >= 0) va = (u32)o;
-                if ((o = se_mod_override(mods, nmods, xe, yb)) >= 0) vb = (u32)o;
+    uint4 a[R], b[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int ya = 2 * (brow0 + r) - offy, yb = ya + 1;
+        const bool live = brow0 + r < brows;
+        a[r] = se_load_quad((live && ya >= 0 && ya < H) ? in + (size_t)ya * pitch : nullptr, x0, W);
+        b[r] = se_load_quad((live && yb < H) ? in + (size_t)yb * pitch : nullptr, x0, W);
+    }
+    u32 ea[R], eb[R];
+    if (offx == 1) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            // cell x0+4 of both rows: lane+1's first cell; lane 63 loads it
+            ea[r] = __shfl_down(a[r].x, 1, 64);
+            eb[r] = __shfl_down(b[r].x, 1, 64);
+            if (lane == 63) {
+                const int ya = 2 * (brow0 + r) - offy, yb = ya + 1;
+                const bool live = brow0 + r < brows;
+                const int xe = base + 256;
+                ea[r] = (live && ya >= 0 && ya < H) ? se_cell_at(in + (size_t)ya * pitch, xe, W) : SE_WALLID;
+                eb[r] = (live && yb < H) ? se_cell_at(in + (size_t)yb * pitch, xe, W) : SE_WALLID;
             }
-            if (ya >= 0) out[(size_t)ya * pitch + xe] = va;
-            if (yb < H) out[(size_t)yb * pitch + xe] = vb;
         }
-        a.x = pa;
-        b.x = pb;
-        if (lane == 0 && strip != 0) {
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int ya = 2 * (brow0 + r) - offy, yb = ya + 1;
+        const bool live = brow0 + r < brows;
+        u32 p0 = a[r].x, p1 = a[r].y, p2 = a[r].z, p3 = a[r].w;
+        u32 q0 = b[r].x, q1 = b[r].y, q2 = b[r].z, q3 = b[r].w;
+        if (offx == 0) {
+            se_block(p0, p1, q0, q1, (u32)x0, (u32)ya, fr);
+            se_block(p2, p3, q2, q3, (u32)(x0 + 2), (u32)ya, fr);
+        } else {
+            u32 e0 = ea[r], e1 = eb[r];
+            u32 edge_a = 0, edge_b = 0;
+            if (strip == 0 && lane == 0) {  // block at x = -1: left column is WALL
+                u32 w0 = SE_WALLID, w2 = SE_WALLID;
+                edge_a = p0;
+                edge_b = q0;
+                se_block(w0, edge_a, w2, edge_b, 0xFFFFFFFFu, (u32)ya, fr);
+            }
+            se_block(p1, p2, q1, q2, (u32)(x0 + 1), (u32)ya, fr);
+            se_block(p3, e0, q3, e1, (u32)(x0 + 3), (u32)ya, fr);
+            // new value of cell x0 comes from lane-1's second block
+            u32 na = __shfl_up(e0, 1, 64), nb = __shfl_up(e1, 1, 64);
+            if (lane == 0) {
+                na = edge_a;
+                nb = edge_b;
+            }
+            p0 = na;
+            q0 = nb;
+            if (lane == 63 && live && base + 256 < W) {  // the cell past the strip
+                const int xe = base + 256;
+                if (nmods > 0) {
+                    int o;
+                    if ((o = se_mod_override(mods, nmods, xe, ya)) >= 0) e0 = (u32)o;
+                    if ((o = se_mod_override(mods, nmods, xe, yb)) >= 0) e1 = (u32)o;
+                }
+                if (ya >= 0) out[(size_t)ya * pitch + xe] = e0;
+                if (yb < H) out[(size_t)yb * pitch + xe] = e1;
+            }
+        }
+        if (!live || x0 >= W) continue;
+        uint4 na = make_uint4(p0, p1, p2, p3), nb = make_uint4(q0, q1, q2, q3);
+        if (nmods > 0 && se_mods_touch(mods, nmods, base, base + 256, ya, yb)) {
+            se_apply_mods_quad(na, mods, nmods, x0, ya);
+            se_apply_mods_quad(nb, mods, nmods, x0, yb);
+        }
+        if (offx == 1 && lane == 0 && strip != 0) {
             // cell x0 belongs to the previous strip's last block: store x0+1..x0+3
-            if (nmods > 0 && se_mods_touch(mods, nmods, x0, x0 + 3, ya, yb)) {
-                se_apply_mods_quad(a, mods, nmods, x0, ya);
-                se_apply_mods_quad(b, mods, nmods, x0, yb);
+            if (ya >= 0) {
+                u32* p = out + (size_t)ya * pitch + x0;
+                p[1] = na.y; p[2] = na.z; p[3] = na.w;
             }
-            if (x0 < W) {
-                if (ya >= 0) {
-                    u32* p = out + (size_t)ya * pitch + x0;
-                    p[1] = a.y; p[2] = a.z; p[3] = a.w;
-                }
-                if (yb < H) {
-                    u32* p = out + (size_t)yb * pitch + x0;
-                    p[1] = b.y; p[2] = b.z; p[3] = b.w;
-                }
+            if (yb < H) {
+                u32* p = out + (size_t)yb * pitch + x0;
+                p[1] = nb.y; p[2] = nb.z; p[3] = nb.w;
             }
-            return;
+            continue;
         }
+        if (ya >= 0) se_store_quad<NT>(out + (size_t)ya * pitch + x0, na);
+        if (yb < H) se_store_quad<NT>(out + (size_t)yb * pitch + x0, nb);
     }
-    if (x0 >= W) return;
-    if (nmods > 0 && se_mods_touch(mods, nmods, base, base + 256, ya, yb)) {
-        se_apply_mods_quad(a, mods, nmods, x0, ya);
-        se_apply_mods_quad(b, mods, nmods, x0, yb);
+}
+
+#define SE_PLAIN_KERNEL(R, NT)                                                                                   \
+    extern "C" __global__ __launch_bounds__(256) void se_step_plain_r##R##_nt##NT(                              \
+        const u32* __restrict__ in, u32* __restrict__ out, int W, int H, int pitch, int frame, int offx,       \
+        int offy, int strips, int brows, const se_mod* __restrict__ mods, int nmods) {                         \
+        se_plain_body<R, NT != 0>(in, out, W, H, pitch, frame, offx, offy, strips, brows, mods, nmods);         \
     }
-    if (ya >= 0) *reinterpret_cast<uint4*>(out + (size_t)ya * pitch + x0) = a;
-    if (yb < H) *reinterpret_cast<uint4*>(out + (size_t)yb * pitch + x0) = b;
+SE_PLAIN_KERNEL(1, 0)
+SE_PLAIN_KERNEL(2, 0)
+SE_PLAIN_KERNEL(4, 0)
+SE_PLAIN_KERNEL(1, 1)
+SE_PLAIN_KERNEL(2, 1)
+SE_PLAIN_KERNEL(4, 1)
+
+// Streaming copy of n4 16-byte quads (the HBM reference for the plain kernel:
+// the same bytes read and written, no rules).
+extern "C" __global__ __launch_bounds__(256) void se_copy(const u32* __restrict__ src, u32* __restrict__ dst,
+                                                          size_t n4) {
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (size_t)gridDim.x * 256)
+        se_store_quad<true>(dst + 4 * i, s[i]);
 }
 
 // Frame 1 of a world clears every cell to EMPTY (falling_sand.glsl:128-131).

@@ -121,7 +121,8 @@ uint64_t fnv1a(const std::string& s) {
 // ------------------------------------------------------------------ internal types
 struct se_module {
     hipModule_t mod = nullptr;
-    hipFunction_t plain = nullptr, fill = nullptr, fill_seeded = nullptr;
+    hipFunction_t plain = nullptr, fill = nullptr, fill_seeded = nullptr, copy = nullptr;
+    hipFunction_t plain_v[6] = {};  // se_step_plain_r{1,2,4}_nt{0,1}
     std::vector<std::pair<se::LdsVariant, hipFunction_t>> lds;
     int device = 0;
     ~se_module() {
@@ -160,6 +161,8 @@ struct se_world {
     std::vector<se_modification> pending;
     void* d_mods = nullptr;
     hipStream_t stream = nullptr;
+    int plain_variant = 3;  // index into se_module::plain_v (default: 1 row pair, non-temporal stores)
+    int lds_variant = -1;   // index into se_module::lds, -1 = default choice
 };
 
 namespace {
@@ -194,9 +197,15 @@ int load_module(se_ctx* ctx, const std::string& src, std::shared_ptr<se_module>&
     auto m = std::make_shared<se_module>();
     m->device = ctx->device;
     HIPCHECK(hipModuleLoadData(&m->mod, code.data()));
-    HIPCHECK(hipModuleGetFunction(&m->plain, m->mod, "se_step_plain"));
+    for (int i = 0; i < 6; i++) {
+        const int R = i % 3 == 0 ? 1 : i % 3 == 1 ? 2 : 4, nt = i / 3;
+        std::string name = "se_step_plain_r" + std::to_string(R) + "_nt" + std::to_string(nt);
+        HIPCHECK(hipModuleGetFunction(&m->plain_v[i], m->mod, name.c_str()));
+    }
+    m->plain = m->plain_v[0];
     HIPCHECK(hipModuleGetFunction(&m->fill, m->mod, "se_fill"));
     HIPCHECK(hipModuleGetFunction(&m->fill_seeded, m->mod, "se_fill_seeded"));
+    HIPCHECK(hipModuleGetFunction(&m->copy, m->mod, "se_copy"));
     for (auto& v : lds_variants()) {
         hipFunction_t f;
         HIPCHECK(hipModuleGetFunction(&f, m->mod, se::lds_kernel_name(v).c_str()));
@@ -260,9 +269,12 @@ int plain_step(se_world* w) {
     offset_of(w->frame, ox, oy);
     int strips = (int)((w->W + 255) / 256);
     int brows = (int)((w->H + (uint32_t)oy + 1) / 2);
-    size_t waves = (size_t)strips * brows;
+    static const int kRows[3] = {1, 2, 4};
+    const int R = kRows[w->plain_variant % 3];
+    size_t waves = (size_t)strips * ((brows + R - 1) / R);
     unsigned blocks = (unsigned)((waves + 3) / 4);
-    int rc2 = launch(w->mod->plain, blocks, w->stream, (const uint32_t*)in, out, (int)w->W, (int)w->H,
+    int rc2 = launch(w->mod->plain_v[w->plain_variant], blocks, w->stream, (const uint32_t*)in, out, (int)w->W,
+                     (int)w->H,
                      (int)w->pitch, (int)w->frame, ox, oy, strips, brows, (const void*)m.ptr, m.n);
     if (rc2) return rc2;
     w->cur ^= 1;
@@ -270,10 +282,14 @@ int plain_step(se_world* w) {
 }
 
 const std::pair<se::LdsVariant, hipFunction_t>* pick_lds(se_world* w, uint32_t sub) {
+    if (w->lds_variant >= 0) {
+        const auto& p = w->mod->lds[w->lds_variant];
+        return (sub == 0 || (uint32_t)p.first.hb >= sub) ? &p : nullptr;
+    }
     const std::pair<se::LdsVariant, hipFunction_t>* best = nullptr;
     for (auto& p : w->mod->lds) {
-        if (sub == 0) {  // default: 256x64 tile, 8 sub-steps
-            if (p.first.tw == 256 && p.first.th == 64 && p.first.hb == 8) return &p;
+        if (sub == 0) {  // default: 128x128 tile, 8 sub-steps
+            if (p.first.tw == 128 && p.first.th == 128 && p.first.hb == 8) return &p;
             continue;
         }
         if ((uint32_t)p.first.hb < sub) continue;
@@ -543,6 +559,33 @@ int se_world_step_blocked(se_world* w, uint32_t nsteps, uint32_t sub) {
     return blocked_steps(w, nsteps, sub);
 }
 
+int se_world_set_variant(se_world* w, int32_t plain_variant, int32_t lds_variant) {
+    if (!w) return fail(SE_ERR_ARG, "null world");
+    if (plain_variant < 0 || plain_variant >= 6) return fail(SE_ERR_ARG, "plain variant out of range");
+    if (lds_variant < -1 || lds_variant >= (int)w->mod->lds.size()) return fail(SE_ERR_ARG, "lds variant out of range");
+    w->plain_variant = plain_variant;
+    w->lds_variant = lds_variant;
+    return SE_OK;
+}
+
+int se_variant_info(const se_world* w, int32_t kind, int32_t index, int32_t* a, int32_t* b, int32_t* c) {
+    if (!w) return fail(SE_ERR_ARG, "null world");
+    if (kind == 0) {
+        if (index < 0 || index >= 6) return fail(SE_ERR_ARG, "plain variant out of range");
+        static const int kRows[3] = {1, 2, 4};
+        if (a) *a = kRows[index % 3];
+        if (b) *b = index / 3;
+        if (c) *c = 0;
+        return SE_OK;
+    }
+    if (index < 0 || index >= (int)w->mod->lds.size()) return fail(SE_ERR_ARG, "lds variant out of range");
+    const se::LdsVariant& v = w->mod->lds[index].first;
+    if (a) *a = v.tw;
+    if (b) *b = v.th;
+    if (c) *c = v.hb;
+    return SE_OK;
+}
+
 int se_world_get_frame(const se_world* w, int32_t* frame) {
     if (!w || !frame) return fail(SE_ERR_ARG, "null argument");
     *frame = w->frame;
@@ -593,6 +636,34 @@ int se_world_fill_seeded(se_world* w, uint64_t seed, const uint32_t* ids, const 
     hipFree(d);
     if (rc) return rc;
     if (e2 != hipSuccess) return fail(SE_ERR_HIP, std::string("fill sync: ") + hipGetErrorString(e2));
+    return SE_OK;
+}
+
+int se_world_copy_timed(se_world* w, uint32_t reps, float* gpu_ms) {
+    if (!w || !gpu_ms) return fail(SE_ERR_ARG, "null argument");
+    DeviceGuard dg(w->ctx->device);
+    size_t n = (size_t)w->pitch * w->H;
+    if (n % 4) return fail(SE_ERR_ARG, "grid size not a multiple of 4 cells");
+    size_t n4 = n / 4;
+    unsigned blocks = (unsigned)std::min<size_t>((n4 + 255) / 256, 1u << 20);
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    hipEventRecord(a, w->stream);
+    int rc = SE_OK;
+    for (uint32_t i = 0; i < reps && !rc; i++) {
+        rc = launch(w->mod->copy, blocks, w->stream, (const uint32_t*)w->buf[w->cur], w->buf[w->cur ^ 1], n4);
+        w->cur ^= 1;
+    }
+    hipEventRecord(b, w->stream);
+    hipError_t e = hipEventSynchronize(b);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(SE_ERR_HIP, hipGetErrorString(e));
+    *gpu_ms = ms;
     return SE_OK;
 }
 

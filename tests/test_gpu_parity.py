@@ -118,7 +118,7 @@ def test_clear_frame_and_api_errors():
         w.push_modification(0, 0, 0, 1, 3)
     with pytest.raises(S.SandengineError):
         w.push_modification(0, 0, 0, 1, 3)
-    with pytest.raises(S.SandengineError):
+    with pytest.raises((S.SandengineError, ValueError)):
         w.upload(np.zeros((3, 3), np.uint32))
     w.close()
 
