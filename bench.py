@@ -194,7 +194,7 @@ def main():
         w.close()
     best = max(results, key=lambda m: results[m]["gcups"])
     R = results[best]
-    kernel = "se_step_plain_r1_nt1" if best == "plain" else "se_step_lds_128x128_h8"
+    kernel = "se_step_plain_r1_nt1" if best == "plain" else "se_step_lds_192x192_h8_t512"
     traffic = load_traffic(kernel)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload)
     out = {

@@ -18,6 +18,8 @@ static const char* kKernelTemplates =
 
 namespace {
 
+constexpr uint64_t kTableMaxBytes = 48 * 1024;  // transition table staged in LDS
+
 [[noreturn]] void gen_error(const std::string& m) { throw LangError("NotRecognized", "(HIP codegen) " + m); }
 
 // ------------------------------------------------------------------ expressions
@@ -450,6 +452,8 @@ public:
     }
 
     std::set<int> lanes_used;
+    std::map<int, std::set<uint32_t>> thresholds;  // lane -> t for every "h <= t" / "h > t" emitted
+    bool rand_float_used = false;                  // rand read as a float (no threshold form)
     bool rank_used = false, dens_used = false, type_used = false;
 
 private:
@@ -490,14 +494,17 @@ private:
             std::string h = "h" + std::to_string(rn->lane);
             if (std::isnan(c)) return op == 5 ? "true" : "false";
             int64_t tle = rand_threshold(c, true), tlt = rand_threshold(c, false);
+            const int lane = rn->lane;
             auto le = [&](int64_t t) -> std::string {  // h <= t
                 if (t < 0) return "false";
                 if (t >= 0xFFFFFFFFll) return "true";
+                thresholds[lane].insert((uint32_t)t);
                 return "(" + h + " <= " + hex_u32((uint64_t)t) + ")";
             };
             auto gt = [&](int64_t t) -> std::string {  // h > t
                 if (t < 0) return "true";
                 if (t >= 0xFFFFFFFFll) return "false";
+                thresholds[lane].insert((uint32_t)t);
                 return "(" + h + " > " + hex_u32((uint64_t)t) + ")";
             };
             switch (op) {
@@ -520,6 +527,7 @@ private:
             case Ex::TypeId: type_used = true; return "((float)se_type(" + std::string(kCell[e.cell]) + "))";
             case Ex::Rand:
                 lanes_used.insert(e.lane);
+                rand_float_used = true;
                 return "((float)h" + std::to_string(e.lane) + " * 0x1p-32f)";
             case Ex::Neg: return "(-" + emit_num(*e.a) + ")";
             default: gen_error("internal: unexpected numeric node");
@@ -731,29 +739,19 @@ std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info) {
     // would select between addresses and keep the cells out of registers)
     o << "#define SE_GSWAP(a, b) do { const unsigned int _a = (a), _b = (b); "
          "const bool _ok = !se_fixed(_a) && !se_fixed(_b); (a) = _ok ? _b : _a; (b) = _ok ? _a : _b; } while (0)\n";
-    // se_block: simulate() for one block (falling_sand.glsl:61-118)
+    // se_block_h: simulate() for one block (falling_sand.glsl:61-118) given the
+    // four hash lanes of the block; se_block computes them from (origin, frame).
     int64_t tmirror = rand_threshold(0.5f, false);
+    em.thresholds[0].insert((uint32_t)tmirror);
     static const unsigned mult[4] = {213u, 2131u, 21313u, 213132u};
     // The rule chain works on local copies, written back once at the end: stores
     // through the reference parameters inside the branches would be merged into
     // stores through pointer phis, which keeps the cells in scratch memory.
-    o << "__device__ __forceinline__ void se_block(unsigned int& r0, unsigned int& r1, unsigned int& r2, "
-         "unsigned int& r3, unsigned int ox, unsigned int oy, unsigned int frame) {\n";
+    o << "__device__ __forceinline__ void se_block_h(unsigned int& r0, unsigned int& r1, unsigned int& r2, "
+         "unsigned int& r3, unsigned int h0, unsigned int h1, unsigned int h2, unsigned int h3) {\n";
+    o << "    (void)h1; (void)h2; (void)h3;\n";
     o << "    unsigned int c0 = r0, c1 = r1, c2 = r2, c3 = r3;\n";
     o << "    if ((c0 | c1 | c2 | c3) == 0u) return;\n";
-    // x = ox*461 + oy*2131 + frame*2131*2131 (math.glsl:74-79) and lane k hashes
-    // x*m_k; the products are distributed so that the per-block terms use 24-bit
-    // multiplies where the constant fits (|ox|, |oy| < 2^23; -1 stays -1).
-    for (int l = 0; l < 4; l++) {
-        if (l > 0 && !em.lanes_used.count(l)) continue;
-        uint32_t A = 461u * mult[l], B = 2131u * mult[l], Cf = 4541161u * mult[l];
-        auto term = [](uint32_t k, const char* v) -> std::string {
-            if (k < (1u << 23)) return "(unsigned int)__mul24((int)" + std::string(v) + ", " + std::to_string(k) + ")";
-            return std::string(v) + " * " + std::to_string(k) + "u";
-        };
-        o << "    const unsigned int h" << l << " = se_lowbias(" << term(A, "ox") << " + " << term(B, "oy")
-          << " + frame * " << Cf << "u);\n";
-    }
     o << "    const bool mirror = h0 <= " << hex_u32((uint64_t)tmirror) << ";\n";
     o << "    if (mirror) { SE_GSWAP(c0, c1); SE_GSWAP(c2, c3); }\n";
     o << mirrored;
@@ -764,6 +762,72 @@ std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info) {
         while (std::getline(is, ln)) o << "    " << ln << "\n";
     }
     o << "    }\n    r0 = c0; r1 = c1; r2 = c2; r3 = c3;\n}\n";
+    // x = ox*461 + oy*2131 + frame*2131*2131 (math.glsl:74-79) and lane k hashes
+    // x*m_k; the products are distributed so that the per-block terms use 24-bit
+    // multiplies where the constant fits (|ox|, |oy| < 2^23; -1 stays -1).
+    auto hash_expr = [&](int l) {
+        uint32_t A = 461u * mult[l], B = 2131u * mult[l], Cf = 4541161u * mult[l];
+        auto term = [](uint32_t k, const char* v) -> std::string {
+            if (k < (1u << 23)) return "(unsigned int)__mul24((int)" + std::string(v) + ", " + std::to_string(k) + ")";
+            return std::string(v) + " * " + std::to_string(k) + "u";
+        };
+        return "se_lowbias(" + term(A, "ox") + " + " + term(B, "oy") + " + frame * " + std::to_string(Cf) + "u)";
+    };
+    o << "__device__ __forceinline__ unsigned int se_hash_lane(int l, unsigned int ox, unsigned int oy, "
+         "unsigned int frame) {\n";
+    for (int l = 0; l < 4; l++) o << "    if (l == " << l << ") return " << hash_expr(l) << ";\n";
+    o << "    return 0u;\n}\n";
+    o << "__device__ __forceinline__ void se_block(unsigned int& r0, unsigned int& r1, unsigned int& r2, "
+         "unsigned int& r3, unsigned int ox, unsigned int oy, unsigned int frame) {\n";
+    o << "    if ((r0 | r1 | r2 | r3) == 0u) return;\n";
+    for (int l = 0; l < 4; l++) {
+        if (l > 0 && !em.lanes_used.count(l)) o << "    const unsigned int h" << l << " = 0u;\n";
+        else o << "    const unsigned int h" << l << " = " << hash_expr(l) << ";\n";
+    }
+    o << "    se_block_h(r0, r1, r2, r3, h0, h1, h2, h3);\n}\n";
+    // Transition table: the block result is a function of the four ids and, per
+    // hash lane, of which threshold interval the lane falls in (every use of a
+    // lane is a compare against one of its thresholds).  Enumerated on the
+    // device by se_build_table from se_block_h itself; used when it fits LDS.
+    {
+        uint64_t R = 1;
+        std::vector<std::vector<uint32_t>> th(4);
+        for (int l = 0; l < 4; l++) {
+            th[l].assign(em.thresholds[l].begin(), em.thresholds[l].end());
+            R *= th[l].size() + 1;
+        }
+        uint64_t tuples = (uint64_t)n * n * n * n;
+        uint64_t bytes = tuples * R * 4;
+        bool ok = !em.rand_float_used && n <= 127 && bytes <= kTableMaxBytes;
+        o << "#define SE_TBL " << (ok ? 1 : 0) << "\n";
+        o << "#define SE_TBL_R " << R << "u\n";
+        o << "#define SE_TBL_ENTRIES " << (ok ? tuples * R : 1) << "u\n";
+        // class index: lane l contributes (#thresholds below h) * stride_l
+        o << "__device__ __forceinline__ unsigned int se_rclass(unsigned int h0, unsigned int h1, unsigned int h2, "
+             "unsigned int h3) {\n    (void)h0; (void)h1; (void)h2; (void)h3;\n    unsigned int r = 0u;\n";
+        uint64_t stride = 1;
+        for (int l = 0; l < 4; l++) {
+            for (uint32_t t : th[l]) o << "    r += (h" << l << " > " << hex_u32(t) << ") ? " << stride << "u : 0u;\n";
+            stride *= th[l].size() + 1;
+        }
+        o << "    return r;\n}\n";
+        // representative hash value of each interval (its upper end)
+        o << "__device__ __forceinline__ unsigned int se_rclass_rep(int l, unsigned int idx) {\n";
+        for (int l = 0; l < 4; l++) {
+            o << "    if (l == " << l << ") {\n";
+            for (size_t j = 0; j < th[l].size(); j++)
+                o << "        if (idx == " << j << "u) return " << hex_u32(th[l][j]) << ";\n";
+            o << "        return 0xffffffffu;\n    }\n";
+        }
+        o << "    return 0u;\n}\n";
+        o << "__device__ __forceinline__ unsigned int se_rclass_count(int l) {\n";
+        for (int l = 0; l < 4; l++) o << "    if (l == " << l << ") return " << th[l].size() + 1 << "u;\n";
+        o << "    return 1u;\n}\n";
+        if (info) {
+            info->table = ok;
+            info->table_R = (int)R;
+        }
+    }
     if (info) {
         info->nmat = n;
         info->nrules = nrules;
@@ -773,7 +837,8 @@ std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info) {
 }
 
 std::string lds_kernel_name(const LdsVariant& v) {
-    return "se_step_lds_" + std::to_string(v.tw) + "x" + std::to_string(v.th) + "_h" + std::to_string(v.hb);
+    return "se_step_lds_" + std::to_string(v.tw) + "x" + std::to_string(v.th) + "_h" + std::to_string(v.hb) + "_t" +
+           std::to_string(v.nt);
 }
 
 std::string hip_program_source(const ParsingResult& r, const std::vector<LdsVariant>& lds) {
@@ -782,7 +847,8 @@ std::string hip_program_source(const ParsingResult& r, const std::vector<LdsVari
     s += kKernelTemplates;
     s += "\n";
     for (auto& v : lds)
-        s += "SE_LDS_KERNEL(" + std::to_string(v.tw) + ", " + std::to_string(v.th) + ", " + std::to_string(v.hb) + ")\n";
+        s += "SE_LDS_KERNEL(" + std::to_string(v.tw) + ", " + std::to_string(v.th) + ", " + std::to_string(v.hb) + ", " +
+             std::to_string(v.nt) + ")\n";
     return s;
 }
 

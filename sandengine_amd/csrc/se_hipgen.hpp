@@ -20,13 +20,15 @@
 namespace se {
 
 struct LdsVariant {
-    int tw, th, hb;
+    int tw, th, hb, nt;  // tile width/height, halo (= max sub-steps), threads per workgroup
 };
 
 struct HipGenInfo {
     int nmat = 0;
     std::vector<int> rand_lanes;  // hash lanes the rule chain reads besides x
     int nrules = 0;
+    bool table = false;  // transition table mode (fits LDS, thresholds only)
+    int table_R = 1;     // rand classes per 4-id tuple
 };
 
 // Generated device code for the rule set (SE_NMAT, se_fixed, se_block, ...).

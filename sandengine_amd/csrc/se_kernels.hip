@@ -252,14 +252,52 @@ extern "C" __global__ __launch_bounds__(256) void se_fill(u32* __restrict__ g, s
 }
 
 // ---------------------------------------------------------------------------
-// LDS temporal-blocked kernel: a workgroup stages a (TW + 2*HB) x (TH + 2*HB)
-// tile (u8 ids) in LDS and advances it `nsub` <= HB Margolus sub-steps in place
-// (blocks partition the cells, so in-place is race-free); each sub-step
-// evaluates only blocks whose four cells are still valid, so the valid region
-// shrinks by at most one cell per side per sub-step and the TW x TH interior is
-// exact after HB sub-steps.  Cells outside the world are WALL in LDS and are
-// never written.  Modifications apply to sub-step 0 only.  Reads `in`, writes
-// the interior to `out` (ping-pong).
+// Transition table (SE_TBL == 1): entry tuple * SE_TBL_R + r holds the block
+// result (four ids packed as bytes c0 | c1<<8 | c2<<16 | c3<<24) for ids
+// tuple = c0 + N*(c1 + N*(c2 + N*c3)) and hash-threshold class r.  Built on the
+// device from se_block_h itself with one representative hash per class.
+// ---------------------------------------------------------------------------
+extern "C" __global__ __launch_bounds__(256) void se_build_table(u32* __restrict__ tbl) {
+    const u32 tuple = blockIdx.x * 256u + threadIdx.x;
+    if (tuple >= SE_TBL_ENTRIES / SE_TBL_R) return;
+    u32 t = tuple;
+    const u32 i0 = t % SE_NMAT;
+    t /= SE_NMAT;
+    const u32 i1 = t % SE_NMAT;
+    t /= SE_NMAT;
+    const u32 i2 = t % SE_NMAT;
+    const u32 i3 = t / SE_NMAT;
+    u32 first = 0, same = 1;
+    for (u32 r = 0; r < SE_TBL_R; r++) {
+        u32 h[4], rr = r;
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            const u32 cnt = se_rclass_count(l);
+            h[l] = se_rclass_rep(l, rr % cnt);
+            rr /= cnt;
+        }
+        u32 c0 = i0, c1 = i1, c2 = i2, c3 = i3;
+        se_block_h(c0, c1, c2, c3, h[0], h[1], h[2], h[3]);
+        const u32 e = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
+        if (r == 0) first = e;
+        same &= (u32)(e == first);
+        tbl[tuple * SE_TBL_R + r] = e;
+    }
+    // bit 31 of the class-0 entry: the result does not depend on the hash (ids < 128)
+    if (same) tbl[tuple * SE_TBL_R] = first | 0x80000000u;
+}
+
+// ---------------------------------------------------------------------------
+// LDS temporal-blocked kernel: a workgroup of NT threads stages a
+// (TW + 2*HB) x (TH + 2*HB) tile (u8 ids) in LDS and advances it `nsub` <= HB
+// Margolus sub-steps in place (blocks partition the cells, so in-place is
+// race-free); each sub-step evaluates only blocks whose four cells are still
+// valid, so the valid region shrinks by at most one cell per side per sub-step
+// and the TW x TH interior is exact after HB sub-steps.  Cells outside the world
+// are WALL in LDS and are never written.  Modifications apply to sub-step 0
+// only.  Reads `in`, writes the interior to `out` (ping-pong).  TW, TH, HB are
+// multiples of 4, so tile rows start 4-byte aligned in LDS and 16-byte aligned
+// in HBM, and the tile moves as 4-cell quads.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void se_offset(u32 f, int& ox, int& oy) {
     const u32 m = f & 3u;  // frame % 4 for frame >= 0
@@ -267,21 +305,41 @@ __device__ __forceinline__ void se_offset(u32 f, int& ox, int& oy) {
     oy = (m == 1u || m == 2u) ? 1 : 0;
 }
 
-template <int TW, int TH, int HB>
+__device__ __forceinline__ u32 se_pack4(u32 a, u32 b, u32 c, u32 d) { return a | (b << 8) | (c << 16) | (d << 24); }
+
+template <int TW, int TH, int HB, int NT>
 __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __restrict__ out, int W, int H,
                                             int pitch, int frame0, int nsub, int tiles_x,
-                                            const se_mod* __restrict__ mods, int nmods, unsigned char* lds) {
-    constexpr int LW = TW + 2 * HB, LH = TH + 2 * HB;
+                                            const se_mod* __restrict__ mods, int nmods,
+                                            const u32* __restrict__ tbl, unsigned char* lds) {
+    constexpr int LW = TW + 2 * HB, LH = TH + 2 * HB, LQ = LW / 4;
     const int tid = threadIdx.x;
     const int tx = (int)blockIdx.x % tiles_x, ty = (int)blockIdx.x / tiles_x;
     const int gx0 = tx * TW - HB, gy0 = ty * TH - HB;
     const bool interior = gx0 >= 0 && gy0 >= 0 && gx0 + LW <= W && gy0 + LH <= H;
-    for (int i = tid; i < LW * LH; i += 256) {
-        const int ly = i / LW, lx = i - ly * LW;
-        const int gx = gx0 + lx, gy = gy0 + ly;
-        u32 v = SE_WALLID;
-        if (gx >= 0 && gx < W && gy >= 0 && gy < H) v = se_decode(in[(size_t)gy * pitch + gx]);
-        lds[i] = (unsigned char)v;
+    u32* lds32 = reinterpret_cast<u32*>(lds);
+#if SE_TBL
+    u32* ltbl = reinterpret_cast<u32*>(lds + ((LW * LH + 15) & ~15));
+    for (int i = tid; i < (int)SE_TBL_ENTRIES; i += NT) ltbl[i] = tbl[i];
+#endif
+    // tile load: 4-cell quads, 16-byte global loads where the quad is inside the world
+    for (int q = tid; q < LQ * LH; q += NT) {
+        const int ly = q / LQ, lq = q - ly * LQ;
+        const int gx = gx0 + 4 * lq, gy = gy0 + ly;
+        u32 v0 = SE_WALLID, v1 = SE_WALLID, v2 = SE_WALLID, v3 = SE_WALLID;
+        if (gy >= 0 && gy < H) {
+            const u32* row = in + (size_t)gy * pitch;
+            if (gx >= 0 && gx + 3 < W) {
+                const uint4 v = *reinterpret_cast<const uint4*>(row + gx);
+                v0 = se_decode(v.x); v1 = se_decode(v.y); v2 = se_decode(v.z); v3 = se_decode(v.w);
+            } else {
+                if (gx >= 0 && gx < W) v0 = se_decode(row[gx]);
+                if (gx + 1 >= 0 && gx + 1 < W) v1 = se_decode(row[gx + 1]);
+                if (gx + 2 >= 0 && gx + 2 < W) v2 = se_decode(row[gx + 2]);
+                if (gx + 3 >= 0 && gx + 3 < W) v3 = se_decode(row[gx + 3]);
+            }
+        }
+        lds32[q] = se_pack4(v0, v1, v2, v3);
     }
     __syncthreads();
     for (int s = 0; s < nsub; s++) {
@@ -289,17 +347,39 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
         int ox, oy;
         se_offset(f, ox, oy);
         const int fx = s + ((s - ox) & 1), fy = s + ((s - oy) & 1);
-        const int nbx = (LW - s - fx) / 2, nby = (LH - s - fy) / 2;  // blocks with fx <= lx, lx+1 <= LW-s-1
+        const int nbx = (LW - s - fx) / 2, nby = (LH - s - fy) / 2;  // blocks with first <= l, l+1 <= L-s-1
         const int nb = nbx * nby;
         const bool use_mods = s == 0 && nmods > 0;
-        for (int b = tid; b < nb; b += 256) {
-            const int by = b / nbx, bx = b - by * nbx;
+        const int dby = NT / nbx, dbx = NT - dby * nbx;
+        int by = tid / nbx, bx = tid - by * nbx;
+        for (int b = tid; b < nb; b += NT) {
             const int lx = fx + 2 * bx, ly = fy + 2 * by;
+            bx += dbx;
+            by += dby;
+            if (bx >= nbx) { bx -= nbx; by += 1; }
             unsigned char* p0 = lds + ly * LW + lx;
             unsigned char* p1 = p0 + LW;
-            u32 c0 = p0[0], c1 = p0[1], c2 = p1[0], c3 = p1[1];
+            u32 c0, c1, c2, c3;
+            if (ox == 0) {  // lx even: 2-byte aligned pairs
+                const u32 u = *reinterpret_cast<const unsigned short*>(p0);
+                const u32 v = *reinterpret_cast<const unsigned short*>(p1);
+                c0 = u & 0xffu; c1 = u >> 8; c2 = v & 0xffu; c3 = v >> 8;
+            } else {
+                c0 = p0[0]; c1 = p0[1]; c2 = p1[0]; c3 = p1[1];
+            }
             const int gx = gx0 + lx, gy = gy0 + ly;
+#if SE_TBL
+            const u32 base = (c0 + SE_NMAT * (c1 + SE_NMAT * (c2 + SE_NMAT * c3))) * SE_TBL_R;
+            u32 e = ltbl[base];
+            if (!(e >> 31)) {  // the result depends on the block's hash classes
+                const u32 r = se_rclass(se_hash_lane(0, (u32)gx, (u32)gy, f), se_hash_lane(1, (u32)gx, (u32)gy, f),
+                                        se_hash_lane(2, (u32)gx, (u32)gy, f), se_hash_lane(3, (u32)gx, (u32)gy, f));
+                e = ltbl[base + r];
+            }
+            c0 = e & 0xffu; c1 = (e >> 8) & 0xffu; c2 = (e >> 16) & 0xffu; c3 = (e >> 24) & 0x7fu;
+#else
             se_block(c0, c1, c2, c3, (u32)gx, (u32)gy, f);
+#endif
             if (use_mods) {
                 int o;
                 if ((o = se_mod_override(mods, nmods, gx, gy)) >= 0) c0 = (u32)o;
@@ -308,8 +388,13 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                 if ((o = se_mod_override(mods, nmods, gx + 1, gy + 1)) >= 0) c3 = (u32)o;
             }
             if (interior) {
-                p0[0] = (unsigned char)c0; p0[1] = (unsigned char)c1;
-                p1[0] = (unsigned char)c2; p1[1] = (unsigned char)c3;
+                if (ox == 0) {
+                    *reinterpret_cast<unsigned short*>(p0) = (unsigned short)(c0 | (c1 << 8));
+                    *reinterpret_cast<unsigned short*>(p1) = (unsigned short)(c2 | (c3 << 8));
+                } else {
+                    p0[0] = (unsigned char)c0; p0[1] = (unsigned char)c1;
+                    p1[0] = (unsigned char)c2; p1[1] = (unsigned char)c3;
+                }
             } else {
                 const bool inx0 = gx >= 0 && gx < W, inx1 = gx + 1 >= 0 && gx + 1 < W;
                 const bool iny0 = gy >= 0 && gy < H, iny1 = gy + 1 >= 0 && gy + 1 < H;
@@ -321,19 +406,32 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
         }
         __syncthreads();
     }
-    for (int i = tid; i < TW * TH; i += 256) {
-        const int ly = i / TW, lx = i - ly * TW;
-        const int gx = gx0 + HB + lx, gy = gy0 + HB + ly;
-        if (gx < W && gy < H) out[(size_t)gy * pitch + gx] = lds[(ly + HB) * LW + lx + HB];
+    // interior store: quads from LDS, 16-byte global stores where inside the world
+    constexpr int TQ = TW / 4;
+    for (int q = tid; q < TQ * TH; q += NT) {
+        const int ly = q / TQ, lq = q - ly * TQ;
+        const int gx = gx0 + HB + 4 * lq, gy = gy0 + HB + ly;
+        if (gy >= H || gx >= W) continue;
+        const u32 p = lds32[((ly + HB) * LW + HB) / 4 + lq];
+        const uint4 v = make_uint4(p & 0xffu, (p >> 8) & 0xffu, (p >> 16) & 0xffu, p >> 24);
+        u32* dst = out + (size_t)gy * pitch + gx;
+        if (gx + 3 < W) {
+            se_store_quad<true>(dst, v);
+        } else {
+            dst[0] = v.x;
+            if (gx + 1 < W) dst[1] = v.y;
+            if (gx + 2 < W) dst[2] = v.z;
+        }
     }
 }
 
-#define SE_LDS_KERNEL(TW, TH, HB)                                                                              \
-    extern "C" __global__ __launch_bounds__(256) void se_step_lds_##TW##x##TH##_h##HB(                        \
+// dynamic LDS: tile bytes rounded to 16, then the table (SE_TBL)
+#define SE_LDS_KERNEL(TW, TH, HB, NT)                                                                         \
+    extern "C" __global__ __launch_bounds__(NT) void se_step_lds_##TW##x##TH##_h##HB##_t##NT(                \
         const u32* __restrict__ in, u32* __restrict__ out, int W, int H, int pitch, int frame0, int nsub,     \
-        int tiles_x, const se_mod* __restrict__ mods, int nmods) {                                             \
-        __shared__ unsigned char lds[(TW + 2 * HB) * (TH + 2 * HB)];                                           \
-        se_lds_body<TW, TH, HB>(in, out, W, H, pitch, frame0, nsub, tiles_x, mods, nmods, lds);               \
+        int tiles_x, const se_mod* __restrict__ mods, int nmods, const u32* __restrict__ tbl) {               \
+        extern __shared__ __align__(16) unsigned char se_lds[];                                               \
+        se_lds_body<TW, TH, HB, NT>(in, out, W, H, pitch, frame0, nsub, tiles_x, mods, nmods, tbl, se_lds);   \
     }
 
 // ---------------------------------------------------------------------------

@@ -47,7 +47,9 @@ char* dupstr(const std::string& s) {
 }
 
 const std::vector<se::LdsVariant>& lds_variants() {
-    static const std::vector<se::LdsVariant> v = {{256, 64, 8}, {128, 128, 8}, {256, 128, 8}, {256, 64, 16}};
+    static const std::vector<se::LdsVariant> v = {
+        {128, 128, 8, 256}, {256, 128, 8, 512}, {128, 128, 16, 256}, {192, 192, 8, 512}, {256, 64, 8, 256},
+        {128, 64, 8, 256}};
     return v;
 }
 
@@ -89,8 +91,9 @@ int parse_yaml(const char* yaml, size_t len, se::ParsingResult& pr, std::string*
     return SE_OK;
 }
 
-int gen_source(const se::ParsingResult& pr, std::string& src) {
+int gen_source(const se::ParsingResult& pr, std::string& src, se::HipGenInfo* info = nullptr) {
     try {
+        if (info) se::hip_rules_source(pr, info);
         src = se::hip_program_source(pr, lds_variants());
     } catch (const se::LangError& e) {
         return fail(SE_ERR_CODEGEN, e.what());
@@ -110,6 +113,14 @@ int launch(hipFunction_t f, unsigned grid, hipStream_t s, A... args) {
     return SE_OK;
 }
 
+template <typename... A>
+int launch_ex(hipFunction_t f, unsigned grid, unsigned block, unsigned shmem, hipStream_t s, A... args) {
+    void* p[] = {(void*)&args...};
+    hipError_t e = hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, shmem, s, p, nullptr);
+    if (e != hipSuccess) return fail(SE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return SE_OK;
+}
+
 uint64_t fnv1a(const std::string& s) {
     uint64_t h = 1469598103934665603ull;
     for (unsigned char c : s) h = (h ^ c) * 1099511628211ull;
@@ -121,6 +132,9 @@ uint64_t fnv1a(const std::string& s) {
 // ------------------------------------------------------------------ internal types
 struct se_module {
     hipModule_t mod = nullptr;
+    uint32_t* table = nullptr;       // transition table (SE_TBL), device memory
+    size_t table_entries = 0;        // 0 when the rule set runs the generated chain
+    std::vector<unsigned> lds_bytes; // dynamic LDS per LDS variant
     hipFunction_t plain = nullptr, fill = nullptr, fill_seeded = nullptr, copy = nullptr;
     hipFunction_t plain_v[6] = {};  // se_step_plain_r{1,2,4}_nt{0,1}
     std::vector<std::pair<se::LdsVariant, hipFunction_t>> lds;
@@ -130,6 +144,7 @@ struct se_module {
             int prev;
             hipGetDevice(&prev);
             hipSetDevice(device);
+            hipFree(table);
             hipModuleUnload(mod);
             hipSetDevice(prev);
         }
@@ -180,7 +195,8 @@ struct DeviceGuard {
     }
 };
 
-int load_module(se_ctx* ctx, const std::string& src, std::shared_ptr<se_module>& out) {
+int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
+                std::shared_ptr<se_module>& out) {
     uint64_t key = fnv1a(src);
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
@@ -206,11 +222,29 @@ int load_module(se_ctx* ctx, const std::string& src, std::shared_ptr<se_module>&
     HIPCHECK(hipModuleGetFunction(&m->fill, m->mod, "se_fill"));
     HIPCHECK(hipModuleGetFunction(&m->fill_seeded, m->mod, "se_fill_seeded"));
     HIPCHECK(hipModuleGetFunction(&m->copy, m->mod, "se_copy"));
+    int max_lds = 65536;
+    hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
+    if (info.table) {
+        size_t n = (size_t)info.nmat;
+        m->table_entries = n * n * n * n * (size_t)info.table_R;
+        HIPCHECK(hipMalloc(&m->table, m->table_entries * 4));
+        hipFunction_t build;
+        HIPCHECK(hipModuleGetFunction(&build, m->mod, "se_build_table"));
+        size_t tuples = n * n * n * n;
+        int rc = launch(build, (unsigned)((tuples + 255) / 256), ctx->stream, m->table);
+        if (rc) return rc;
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
     for (auto& v : lds_variants()) {
         hipFunction_t f;
         HIPCHECK(hipModuleGetFunction(&f, m->mod, se::lds_kernel_name(v).c_str()));
+        unsigned bytes = (unsigned)(((v.tw + 2 * v.hb) * (v.th + 2 * v.hb) + 15) & ~15) +
+                         (unsigned)(m->table_entries * 4);
+        if ((int)bytes > max_lds) continue;  // does not fit this device's LDS
         m->lds.push_back({v, f});
+        m->lds_bytes.push_back(bytes);
     }
+    if (m->lds.empty()) return fail(SE_ERR_JIT, "no LDS kernel variant fits the device's LDS");
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->cache[key] = {src, m};
     out = m;
@@ -281,21 +315,22 @@ int plain_step(se_world* w) {
     return SE_OK;
 }
 
+// LDS instantiation for a launch of up to `sub` sub-steps (0: the variant's halo):
+// the world's tuned choice if set, else the 192x192 / halo 8 / 512-thread tile
+// (fastest measured on a 16384^2 world) when it allows `sub`, else the variant
+// with the smallest halo >= sub.
 const std::pair<se::LdsVariant, hipFunction_t>* pick_lds(se_world* w, uint32_t sub) {
+    auto& L = w->mod->lds;
     if (w->lds_variant >= 0) {
-        const auto& p = w->mod->lds[w->lds_variant];
+        const auto& p = L[w->lds_variant];
         return (sub == 0 || (uint32_t)p.first.hb >= sub) ? &p : nullptr;
     }
+    for (auto& p : L)
+        if (p.first.tw == 192 && p.first.th == 192 && p.first.hb == 8 && (sub == 0 || sub <= 8)) return &p;
     const std::pair<se::LdsVariant, hipFunction_t>* best = nullptr;
-    for (auto& p : w->mod->lds) {
-        if (sub == 0) {  // default: 128x128 tile, 8 sub-steps
-            if (p.first.tw == 128 && p.first.th == 128 && p.first.hb == 8) return &p;
-            continue;
-        }
-        if ((uint32_t)p.first.hb < sub) continue;
-        if (!best || p.first.hb < best->first.hb ||
-            (p.first.hb == best->first.hb && p.first.tw * p.first.th > best->first.tw * best->first.th))
-            best = &p;
+    for (auto& p : L) {
+        if (sub != 0 && (uint32_t)p.first.hb < sub) continue;
+        if (!best || p.first.hb < best->first.hb) best = &p;
     }
     return best;
 }
@@ -307,9 +342,10 @@ int lds_launch(se_world* w, const std::pair<se::LdsVariant, hipFunction_t>& v, i
     if (rc) return rc;
     const se::LdsVariant& L = v.first;
     int tiles_x = (int)((w->W + L.tw - 1) / L.tw), tiles_y = (int)((w->H + L.th - 1) / L.th);
-    rc = launch(v.second, (unsigned)(tiles_x * tiles_y), w->stream, (const uint32_t*)w->buf[w->cur],
-                w->buf[w->cur ^ 1], (int)w->W, (int)w->H, (int)w->pitch, (int)(w->frame + 1), n, tiles_x,
-                (const void*)m.ptr, m.n);
+    size_t vi = &v - &w->mod->lds[0];
+    rc = launch_ex(v.second, (unsigned)(tiles_x * tiles_y), (unsigned)L.nt, w->mod->lds_bytes[vi], w->stream,
+                   (const uint32_t*)w->buf[w->cur], w->buf[w->cur ^ 1], (int)w->W, (int)w->H, (int)w->pitch,
+                   (int)(w->frame + 1), n, tiles_x, (const void*)m.ptr, m.n, (const uint32_t*)w->mod->table);
     if (rc) return rc;
     w->frame += n;
     w->cur ^= 1;
@@ -438,9 +474,10 @@ int se_ruleset_from_yaml(se_ctx* ctx, const char* yaml, size_t len, se_ruleset**
     int rc = parse_yaml(yaml, len, rs->pr);
     if (rc) return rc;
     std::string src;
-    rc = gen_source(rs->pr, src);
+    se::HipGenInfo info;
+    rc = gen_source(rs->pr, src, &info);
     if (rc) return rc;
-    rc = load_module(ctx, src, rs->mod);
+    rc = load_module(ctx, src, info, rs->mod);
     if (rc) return rc;
     rs->ctx = ctx;
     rs->nmat = (int)rs->pr.materials.size();
@@ -582,7 +619,7 @@ int se_variant_info(const se_world* w, int32_t kind, int32_t index, int32_t* a, 
     const se::LdsVariant& v = w->mod->lds[index].first;
     if (a) *a = v.tw;
     if (b) *b = v.th;
-    if (c) *c = v.hb;
+    if (c) *c = v.hb * 1000 + v.nt;
     return SE_OK;
 }
 

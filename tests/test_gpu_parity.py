@@ -174,3 +174,47 @@ def test_large_plain_equals_blocked_and_conserves(side, steps):
     np.testing.assert_array_equal(np.bincount(ga.ravel(), minlength=6), counts)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("rsname", ["four", "twelve", "features"])
+def test_every_kernel_variant_matches_oracle(rsname):
+    """All plain and LDS instantiations (rows per wave, store kind, tile, halo,
+    threads) give the oracle's grid, with modifications in the run."""
+    print(f"\n[gpu] variants {rsname}", flush=True)
+    rs = ruleset(rsname)
+    blob = O.compile_ruleset(L.parse_string(load.ruleset_text(rsname)))
+    W, H, steps, first = 530, 301, 35, 6
+    case = dict(ruleset=rsname, W=W, H=H, seed=77, specials=0.01)
+    g0 = C.make_input(case)
+    rng = np.random.RandomState(5)
+    mods_at = {0: C._rand_mods(rng, W, H, C.NMAT[rsname], 4), 17: C._rand_mods(rng, W, H, C.NMAT[rsname], 3)}
+    want = g0
+    for s in range(steps):
+        want = O.step(blob, want, first + s, mods_at.get(s))
+    probe = S.World(ctx(), rs, 8, 8)
+    n_lds = 0
+    while True:
+        try:
+            probe.variant_info(1, n_lds)
+        except S.SandengineError:
+            break
+        n_lds += 1
+    probe.close()
+    assert n_lds >= 3
+    for pv in range(6):
+        for lv, sub in [(-1, 0)] + [(i, s) for i in range(n_lds) for s in (0, 3)] if pv == 3 else [(-1, 0)]:
+            w = S.World(ctx(), rs, W, H)
+            w.set_variant(pv, lv)
+            w.step(1)
+            w.upload(g0)
+            w.frame = first - 1
+            stops = [0, 17, steps]
+            for a, b in zip(stops[:-1], stops[1:]):
+                for m in mods_at.get(a, []):
+                    w.push_modification(*m)
+                if lv < 0:
+                    w.step(b - a)
+                else:
+                    w.step_blocked(b - a, sub)
+            np.testing.assert_array_equal(w.download(), want, err_msg=f"plain {pv} lds {lv} sub {sub}")
+            w.close()
