@@ -139,6 +139,29 @@ int se_world_step_timed(se_world* w, uint32_t nsteps, int mode, uint32_t sub, fl
  * reference for the plain kernel.  Leaves the grid contents unchanged. */
 int se_world_copy_timed(se_world* w, uint32_t reps, float* gpu_ms);
 
+/* ---------------------------------------------------------------- batched worlds
+ * No reference counterpart (each reference Simulation is one world,
+ * simulation.rs:97-126); every world in a batch takes exactly the steps
+ * se_world_step would give it. */
+int se_batch_create(se_ctx* ctx, se_batch** out);
+void se_batch_destroy(se_batch* b);
+/* The world advances `nsteps` steps in each following se_batch_run. */
+int se_batch_add(se_batch* b, se_world* w, uint32_t nsteps);
+int se_batch_clear(se_batch* b);
+/* mode 0: loop the plain kernel world by world (one launch per step);
+ * mode 1: batched persistent LDS kernel -- worlds that fit one tile advance
+ * through all their steps in one work item, larger ones in rounds of k
+ * sub-steps, rule-set groups and the two kinds overlapping on separate streams.
+ * lds_variant -1 / k 0: the autotuned (else default) choice; graph != 0
+ * captures the whole submission in a hipGraph and launches that.
+ * *gpu_ms: time between events around the submission. */
+int se_batch_run(se_batch* b, int mode, int32_t lds_variant, uint32_t k, int graph, float* gpu_ms);
+/* Autotuner: for each rule-set group, times every LDS variant with k = halo and
+ * k = halo/2 on `rounds` rounds of the batch's own worlds (grids and frames are
+ * snapshotted and restored) and keeps the fastest for later runs. */
+int se_batch_autotune(se_batch* b, uint32_t rounds, int32_t* best_variant, uint32_t* best_k,
+                      float* best_ns_per_kcell);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,6 +105,13 @@ def lib():
         L.se_world_step_timed.argtypes = [c_p, c_u32, ctypes.c_int, c_u32, ctypes.POINTER(ctypes.c_float)]
         L.se_world_set_variant.argtypes = [c_p, c_i32, c_i32]
         L.se_world_copy_timed.argtypes = [c_p, c_u32, ctypes.POINTER(ctypes.c_float)]
+        L.se_batch_create.argtypes = [c_p, PP]
+        L.se_batch_destroy.argtypes = [c_p]
+        L.se_batch_add.argtypes = [c_p, c_p, c_u32]
+        L.se_batch_clear.argtypes = [c_p]
+        L.se_batch_run.argtypes = [c_p, ctypes.c_int, c_i32, c_u32, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+        L.se_batch_autotune.argtypes = [c_p, c_u32, ctypes.POINTER(c_i32), ctypes.POINTER(c_u32),
+                                        ctypes.POINTER(ctypes.c_float)]
         L.se_variant_info.argtypes = [c_p, c_i32, c_i32] + [ctypes.POINTER(c_i32)] * 3
         _lib = L
     return _lib
@@ -368,6 +375,45 @@ class World:
     def close(self):
         if getattr(self, "_p", None) and self._p.value:
             lib().se_world_destroy(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many worlds advanced together (se_batch): looped plain kernel or the
+    batched persistent LDS kernel, optionally through one hipGraph."""
+
+    def __init__(self, ctx: Context):
+        p = ctypes.c_void_p()
+        _check(lib().se_batch_create(ctx._p, ctypes.byref(p)))
+        self._p, self.ctx, self.worlds = p, ctx, []
+
+    def add(self, world: World, nsteps: int):
+        _check(lib().se_batch_add(self._p, world._p, nsteps))
+        self.worlds.append((world, nsteps))
+
+    def clear(self):
+        _check(lib().se_batch_clear(self._p))
+        self.worlds = []
+
+    def run(self, mode: int = 1, variant: int = -1, k: int = 0, graph: bool = False) -> float:
+        ms = ctypes.c_float()
+        _check(lib().se_batch_run(self._p, mode, variant, k, 1 if graph else 0, ctypes.byref(ms)))
+        return ms.value
+
+    def autotune(self, rounds: int = 3) -> Tuple[int, int, float]:
+        v, k, s = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_float()
+        _check(lib().se_batch_autotune(self._p, rounds, ctypes.byref(v), ctypes.byref(k), ctypes.byref(s)))
+        return v.value, k.value, s.value
+
+    def close(self):
+        if getattr(self, "_p", None) and self._p.value:
+            lib().se_batch_destroy(self._p)
             self._p = ctypes.c_void_p()
 
     def __del__(self):

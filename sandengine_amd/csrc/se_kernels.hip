@@ -307,111 +307,134 @@ __device__ __forceinline__ void se_offset(u32 f, int& ox, int& oy) {
 
 __device__ __forceinline__ u32 se_pack4(u32 a, u32 b, u32 c, u32 d) { return a | (b << 8) | (c << 16) | (d << 24); }
 
+// Block-origin range along one axis at sub-step s: origins o (local) with parity
+// `par`, whose block touches the world (g0 + o in [-1, n-1]) and lies in the valid
+// region.  A side where the LDS tile cuts into the world loses one cell of
+// validity per sub-step; a side where the tile holds the world's WALL border
+// does not (those cells never change).
+__device__ __forceinline__ void se_block_range(int s, int par, int g0, int L, int n, int& lo, int& cnt) {
+    const bool open_lo = g0 > -1, open_hi = g0 + L < n + 1;
+    int a = open_lo ? s : 0;
+    const int wa = -1 - g0;
+    if (wa > a) a = wa;
+    if ((a - par) & 1) a++;
+    int b = (open_hi ? L - s : L) - 2;
+    const int wb = n - 1 - g0;
+    if (wb < b) b = wb;
+    lo = a;
+    cnt = b >= a ? (b - a) / 2 + 1 : 0;
+}
+
 template <int TW, int TH, int HB, int NT>
 __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __restrict__ out, int W, int H,
-                                            int pitch, int frame0, int nsub, int tiles_x,
+                                            int pitch, int frame0, int nsub, int tiles_x, int tile,
                                             const se_mod* __restrict__ mods, int nmods,
-                                            const u32* __restrict__ tbl, unsigned char* lds) {
+                                            const u32* __restrict__ ltbl, unsigned char* lds) {
     constexpr int LW = TW + 2 * HB, LH = TH + 2 * HB, LQ = LW / 4;
     const int tid = threadIdx.x;
-    const int tx = (int)blockIdx.x % tiles_x, ty = (int)blockIdx.x / tiles_x;
+    const int tx = tile % tiles_x, ty = tile / tiles_x;
     const int gx0 = tx * TW - HB, gy0 = ty * TH - HB;
     const bool interior = gx0 >= 0 && gy0 >= 0 && gx0 + LW <= W && gy0 + LH <= H;
     u32* lds32 = reinterpret_cast<u32*>(lds);
-#if SE_TBL
-    u32* ltbl = reinterpret_cast<u32*>(lds + ((LW * LH + 15) & ~15));
-    for (int i = tid; i < (int)SE_TBL_ENTRIES; i += NT) ltbl[i] = tbl[i];
-#endif
-    // tile load: 4-cell quads, 16-byte global loads where the quad is inside the world
-    for (int q = tid; q < LQ * LH; q += NT) {
-        const int ly = q / LQ, lq = q - ly * LQ;
-        const int gx = gx0 + 4 * lq, gy = gy0 + ly;
-        u32 v0 = SE_WALLID, v1 = SE_WALLID, v2 = SE_WALLID, v3 = SE_WALLID;
-        if (gy >= 0 && gy < H) {
-            const u32* row = in + (size_t)gy * pitch;
-            if (gx >= 0 && gx + 3 < W) {
-                const uint4 v = *reinterpret_cast<const uint4*>(row + gx);
-                v0 = se_decode(v.x); v1 = se_decode(v.y); v2 = se_decode(v.z); v3 = se_decode(v.w);
-            } else {
-                if (gx >= 0 && gx < W) v0 = se_decode(row[gx]);
-                if (gx + 1 >= 0 && gx + 1 < W) v1 = se_decode(row[gx + 1]);
-                if (gx + 2 >= 0 && gx + 2 < W) v2 = se_decode(row[gx + 2]);
-                if (gx + 3 >= 0 && gx + 3 < W) v3 = se_decode(row[gx + 3]);
+    // tile load: rows/quads that touch the world plus its one-cell WALL border
+    {
+        const int ly_lo = max(0, -1 - gy0), ly_hi = min(LH, H + 1 - gy0);
+        const int lq_lo = max(0, (-1 - gx0) >> 2), lq_hi = min(LQ, (W + 1 - gx0 + 3) >> 2);
+        const int nq = max(0, lq_hi - lq_lo), nr = max(0, ly_hi - ly_lo);
+        for (int i = tid; i < nq * nr; i += NT) {
+            const int ry = i / nq;
+            const int ly = ly_lo + ry, lq = lq_lo + (i - ry * nq);
+            const int gx = gx0 + 4 * lq, gy = gy0 + ly;
+            u32 v0 = SE_WALLID, v1 = SE_WALLID, v2 = SE_WALLID, v3 = SE_WALLID;
+            if (gy >= 0 && gy < H) {
+                const u32* row = in + (size_t)gy * pitch;
+                if (gx >= 0 && gx + 3 < W) {
+                    const uint4 v = *reinterpret_cast<const uint4*>(row + gx);
+                    v0 = se_decode(v.x); v1 = se_decode(v.y); v2 = se_decode(v.z); v3 = se_decode(v.w);
+                } else {
+                    if (gx >= 0 && gx < W) v0 = se_decode(row[gx]);
+                    if (gx + 1 >= 0 && gx + 1 < W) v1 = se_decode(row[gx + 1]);
+                    if (gx + 2 >= 0 && gx + 2 < W) v2 = se_decode(row[gx + 2]);
+                    if (gx + 3 >= 0 && gx + 3 < W) v3 = se_decode(row[gx + 3]);
+                }
             }
+            lds32[ly * LQ + lq] = se_pack4(v0, v1, v2, v3);
         }
-        lds32[q] = se_pack4(v0, v1, v2, v3);
     }
     __syncthreads();
     for (int s = 0; s < nsub; s++) {
         const u32 f = (u32)(frame0 + s);
         int ox, oy;
         se_offset(f, ox, oy);
-        const int fx = s + ((s - ox) & 1), fy = s + ((s - oy) & 1);
-        const int nbx = (LW - s - fx) / 2, nby = (LH - s - fy) / 2;  // blocks with first <= l, l+1 <= L-s-1
+        int fx, nbx, fy, nby;
+        se_block_range(s, ox, gx0, LW, W, fx, nbx);
+        se_block_range(s, oy, gy0, LH, H, fy, nby);
         const int nb = nbx * nby;
         const bool use_mods = s == 0 && nmods > 0;
-        const int dby = NT / nbx, dbx = NT - dby * nbx;
-        int by = tid / nbx, bx = tid - by * nbx;
-        for (int b = tid; b < nb; b += NT) {
-            const int lx = fx + 2 * bx, ly = fy + 2 * by;
-            bx += dbx;
-            by += dby;
-            if (bx >= nbx) { bx -= nbx; by += 1; }
-            unsigned char* p0 = lds + ly * LW + lx;
-            unsigned char* p1 = p0 + LW;
-            u32 c0, c1, c2, c3;
-            if (ox == 0) {  // lx even: 2-byte aligned pairs
-                const u32 u = *reinterpret_cast<const unsigned short*>(p0);
-                const u32 v = *reinterpret_cast<const unsigned short*>(p1);
-                c0 = u & 0xffu; c1 = u >> 8; c2 = v & 0xffu; c3 = v >> 8;
-            } else {
-                c0 = p0[0]; c1 = p0[1]; c2 = p1[0]; c3 = p1[1];
-            }
-            const int gx = gx0 + lx, gy = gy0 + ly;
-#if SE_TBL
-            const u32 base = (c0 + SE_NMAT * (c1 + SE_NMAT * (c2 + SE_NMAT * c3))) * SE_TBL_R;
-            u32 e = ltbl[base];
-            if (!(e >> 31)) {  // the result depends on the block's hash classes
-                const u32 r = se_rclass(se_hash_lane(0, (u32)gx, (u32)gy, f), se_hash_lane(1, (u32)gx, (u32)gy, f),
-                                        se_hash_lane(2, (u32)gx, (u32)gy, f), se_hash_lane(3, (u32)gx, (u32)gy, f));
-                e = ltbl[base + r];
-            }
-            c0 = e & 0xffu; c1 = (e >> 8) & 0xffu; c2 = (e >> 16) & 0xffu; c3 = (e >> 24) & 0x7fu;
-#else
-            se_block(c0, c1, c2, c3, (u32)gx, (u32)gy, f);
-#endif
-            if (use_mods) {
-                int o;
-                if ((o = se_mod_override(mods, nmods, gx, gy)) >= 0) c0 = (u32)o;
-                if ((o = se_mod_override(mods, nmods, gx + 1, gy)) >= 0) c1 = (u32)o;
-                if ((o = se_mod_override(mods, nmods, gx, gy + 1)) >= 0) c2 = (u32)o;
-                if ((o = se_mod_override(mods, nmods, gx + 1, gy + 1)) >= 0) c3 = (u32)o;
-            }
-            if (interior) {
-                if (ox == 0) {
-                    *reinterpret_cast<unsigned short*>(p0) = (unsigned short)(c0 | (c1 << 8));
-                    *reinterpret_cast<unsigned short*>(p1) = (unsigned short)(c2 | (c3 << 8));
+        if (nb > 0) {
+            const int dby = NT / nbx, dbx = NT - dby * nbx;
+            int by = tid / nbx, bx = tid - by * nbx;
+            for (int b = tid; b < nb; b += NT) {
+                const int lx = fx + 2 * bx, ly = fy + 2 * by;
+                bx += dbx;
+                by += dby;
+                if (bx >= nbx) { bx -= nbx; by += 1; }
+                unsigned char* p0 = lds + ly * LW + lx;
+                unsigned char* p1 = p0 + LW;
+                u32 c0, c1, c2, c3;
+                if (ox == 0) {  // lx even: 2-byte aligned pairs
+                    const u32 u = *reinterpret_cast<const unsigned short*>(p0);
+                    const u32 v = *reinterpret_cast<const unsigned short*>(p1);
+                    c0 = u & 0xffu; c1 = u >> 8; c2 = v & 0xffu; c3 = v >> 8;
                 } else {
-                    p0[0] = (unsigned char)c0; p0[1] = (unsigned char)c1;
-                    p1[0] = (unsigned char)c2; p1[1] = (unsigned char)c3;
+                    c0 = p0[0]; c1 = p0[1]; c2 = p1[0]; c3 = p1[1];
                 }
-            } else {
-                const bool inx0 = gx >= 0 && gx < W, inx1 = gx + 1 >= 0 && gx + 1 < W;
-                const bool iny0 = gy >= 0 && gy < H, iny1 = gy + 1 >= 0 && gy + 1 < H;
-                if (inx0 && iny0) p0[0] = (unsigned char)c0;
-                if (inx1 && iny0) p0[1] = (unsigned char)c1;
-                if (inx0 && iny1) p1[0] = (unsigned char)c2;
-                if (inx1 && iny1) p1[1] = (unsigned char)c3;
+                const int gx = gx0 + lx, gy = gy0 + ly;
+#if SE_TBL
+                const u32 base = (c0 + SE_NMAT * (c1 + SE_NMAT * (c2 + SE_NMAT * c3))) * SE_TBL_R;
+                u32 e = ltbl[base];
+                if (!(e >> 31)) {  // the result depends on the block's hash classes
+                    const u32 r = se_rclass(se_hash_lane(0, (u32)gx, (u32)gy, f), se_hash_lane(1, (u32)gx, (u32)gy, f),
+                                            se_hash_lane(2, (u32)gx, (u32)gy, f), se_hash_lane(3, (u32)gx, (u32)gy, f));
+                    e = ltbl[base + r];
+                }
+                c0 = e & 0xffu; c1 = (e >> 8) & 0xffu; c2 = (e >> 16) & 0xffu; c3 = (e >> 24) & 0x7fu;
+#else
+                se_block(c0, c1, c2, c3, (u32)gx, (u32)gy, f);
+#endif
+                if (use_mods) {
+                    int o;
+                    if ((o = se_mod_override(mods, nmods, gx, gy)) >= 0) c0 = (u32)o;
+                    if ((o = se_mod_override(mods, nmods, gx + 1, gy)) >= 0) c1 = (u32)o;
+                    if ((o = se_mod_override(mods, nmods, gx, gy + 1)) >= 0) c2 = (u32)o;
+                    if ((o = se_mod_override(mods, nmods, gx + 1, gy + 1)) >= 0) c3 = (u32)o;
+                }
+                if (interior) {
+                    if (ox == 0) {
+                        *reinterpret_cast<unsigned short*>(p0) = (unsigned short)(c0 | (c1 << 8));
+                        *reinterpret_cast<unsigned short*>(p1) = (unsigned short)(c2 | (c3 << 8));
+                    } else {
+                        p0[0] = (unsigned char)c0; p0[1] = (unsigned char)c1;
+                        p1[0] = (unsigned char)c2; p1[1] = (unsigned char)c3;
+                    }
+                } else {
+                    const bool inx0 = gx >= 0 && gx < W, inx1 = gx + 1 >= 0 && gx + 1 < W;
+                    const bool iny0 = gy >= 0 && gy < H, iny1 = gy + 1 >= 0 && gy + 1 < H;
+                    if (inx0 && iny0) p0[0] = (unsigned char)c0;
+                    if (inx1 && iny0) p0[1] = (unsigned char)c1;
+                    if (inx0 && iny1) p1[0] = (unsigned char)c2;
+                    if (inx1 && iny1) p1[1] = (unsigned char)c3;
+                }
             }
         }
         __syncthreads();
     }
     // interior store: quads from LDS, 16-byte global stores where inside the world
     constexpr int TQ = TW / 4;
-    for (int q = tid; q < TQ * TH; q += NT) {
-        const int ly = q / TQ, lq = q - ly * TQ;
+    const int sy_hi = max(0, min(TH, H - (gy0 + HB))), sq_hi = max(0, min(TQ, (W - (gx0 + HB) + 3) >> 2));
+    for (int i = tid; i < sq_hi * sy_hi; i += NT) {
+        const int ly = i / sq_hi, lq = i - ly * sq_hi;
         const int gx = gx0 + HB + 4 * lq, gy = gy0 + HB + ly;
-        if (gy >= H || gx >= W) continue;
         const u32 p = lds32[((ly + HB) * LW + HB) / 4 + lq];
         const uint4 v = make_uint4(p & 0xffu, (p >> 8) & 0xffu, (p >> 16) & 0xffu, p >> 24);
         u32* dst = out + (size_t)gy * pitch + gx;
@@ -423,15 +446,72 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
             if (gx + 2 < W) dst[2] = v.z;
         }
     }
+    __syncthreads();  // LDS is reused by the next item of a persistent workgroup
+}
+
+__device__ __forceinline__ const u32* se_stage_table(const u32* __restrict__ tbl, unsigned char* lds, int lds_tile,
+                                                     int nt) {
+#if SE_TBL
+    u32* ltbl = reinterpret_cast<u32*>(lds + ((lds_tile + 15) & ~15));
+    for (int i = threadIdx.x; i < (int)SE_TBL_ENTRIES; i += nt) ltbl[i] = tbl[i];
+    __syncthreads();
+    return ltbl;
+#else
+    (void)tbl; (void)lds; (void)lds_tile; (void)nt;
+    return nullptr;
+#endif
+}
+
+// Per-world record of a batched run (device memory, filled by the host).
+struct se_bworld {
+    u32* buf[2];     // the world's two grids; buf[cur0] holds the grid at batch start
+    int W, H, pitch, tiles_x, ntiles;
+    int frame0;      // frame of the world's first step in this run
+    int steps;       // steps the world advances in this run
+    int cur0;
+};
+
+// Index of the world owning item `item`: prefix[w] <= item < prefix[w+1].
+__device__ __forceinline__ int se_owner(const int* __restrict__ prefix, int n, int item) {
+    int lo = 0, hi = n;  // prefix[lo] <= item < prefix[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
 }
 
 // dynamic LDS: tile bytes rounded to 16, then the table (SE_TBL)
+//  se_step_lds_*  : one world, one workgroup per tile.
+//  se_batch_lds_* : persistent over many worlds: workgroups stride over the
+//                   tiles of worlds [0, n_active) (sorted by steps, longest first)
+//                   and advance each by min(k, steps - round*k) sub-steps; a world
+//                   that fits one tile runs with k >= all its steps in one item.
 #define SE_LDS_KERNEL(TW, TH, HB, NT)                                                                         \
     extern "C" __global__ __launch_bounds__(NT) void se_step_lds_##TW##x##TH##_h##HB##_t##NT(                \
         const u32* __restrict__ in, u32* __restrict__ out, int W, int H, int pitch, int frame0, int nsub,     \
         int tiles_x, const se_mod* __restrict__ mods, int nmods, const u32* __restrict__ tbl) {               \
         extern __shared__ __align__(16) unsigned char se_lds[];                                               \
-        se_lds_body<TW, TH, HB, NT>(in, out, W, H, pitch, frame0, nsub, tiles_x, mods, nmods, tbl, se_lds);   \
+        const u32* lt = se_stage_table(tbl, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT);                      \
+        se_lds_body<TW, TH, HB, NT>(in, out, W, H, pitch, frame0, nsub, tiles_x, (int)blockIdx.x, mods, nmods, \
+                                    lt, se_lds);                                                              \
+    }                                                                                                         \
+    extern "C" __global__ __launch_bounds__(NT) void se_batch_lds_##TW##x##TH##_h##HB##_t##NT(               \
+        const se_bworld* __restrict__ worlds, const int* __restrict__ prefix, int n_active, int round, int k, \
+        const u32* __restrict__ tbl) {                                                                        \
+        extern __shared__ __align__(16) unsigned char se_lds[];                                               \
+        const u32* lt = se_stage_table(tbl, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT);                      \
+        const int n_items = prefix[n_active];                                                                 \
+        for (int item = (int)blockIdx.x; item < n_items; item += (int)gridDim.x) {                            \
+            const int w = se_owner(prefix, n_active, item);                                                   \
+            const se_bworld& B = worlds[w];                                                                   \
+            const int done = round * k;                                                                       \
+            const int nsub = min(k, B.steps - done);                                                          \
+            const int cur = (B.cur0 + round) & 1;                                                             \
+            se_lds_body<TW, TH, HB, NT>(B.buf[cur], B.buf[cur ^ 1], B.W, B.H, B.pitch, B.frame0 + done, nsub,  \
+                                        B.tiles_x, item - prefix[w], nullptr, 0, lt, se_lds);                 \
+        }                                                                                                     \
     }
 
 // ---------------------------------------------------------------------------

@@ -21,24 +21,24 @@
 #include <string>
 #include <vector>
 
-#include "../../include/sandengine_hip.h"
-#include "se_hipgen.hpp"
-#include "se_lang.hpp"
+#include "se_internal.hpp"
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
+namespace se_rt {
 int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
+}  // namespace se_rt
 
-#define HIPCHECK(expr)                                                                                 \
-    do {                                                                                               \
-        hipError_t _e = (expr);                                                                        \
-        if (_e != hipSuccess) return fail(SE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
-    } while (0)
+namespace {
+using se_rt::fail;
+using se_rt::launch;
+using se_rt::launch_ex;
+using se_rt::DeviceGuard;
 
 char* dupstr(const std::string& s) {
     char* p = (char*)std::malloc(s.size() + 1);
@@ -103,24 +103,6 @@ int gen_source(const se::ParsingResult& pr, std::string& src, se::HipGenInfo* in
     return SE_OK;
 }
 
-// Launches `f` on a 1-D grid of 256-thread workgroups; each argument is passed
-// by address (kernelParams), so its C++ type must match the kernel parameter.
-template <typename... A>
-int launch(hipFunction_t f, unsigned grid, hipStream_t s, A... args) {
-    void* p[] = {(void*)&args...};
-    hipError_t e = hipModuleLaunchKernel(f, grid, 1, 1, 256, 1, 1, 0, s, p, nullptr);
-    if (e != hipSuccess) return fail(SE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
-    return SE_OK;
-}
-
-template <typename... A>
-int launch_ex(hipFunction_t f, unsigned grid, unsigned block, unsigned shmem, hipStream_t s, A... args) {
-    void* p[] = {(void*)&args...};
-    hipError_t e = hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, shmem, s, p, nullptr);
-    if (e != hipSuccess) return fail(SE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
-    return SE_OK;
-}
-
 uint64_t fnv1a(const std::string& s) {
     uint64_t h = 1469598103934665603ull;
     for (unsigned char c : s) h = (h ^ c) * 1099511628211ull;
@@ -129,71 +111,7 @@ uint64_t fnv1a(const std::string& s) {
 
 }  // namespace
 
-// ------------------------------------------------------------------ internal types
-struct se_module {
-    hipModule_t mod = nullptr;
-    uint32_t* table = nullptr;       // transition table (SE_TBL), device memory
-    size_t table_entries = 0;        // 0 when the rule set runs the generated chain
-    std::vector<unsigned> lds_bytes; // dynamic LDS per LDS variant
-    hipFunction_t plain = nullptr, fill = nullptr, fill_seeded = nullptr, copy = nullptr;
-    hipFunction_t plain_v[6] = {};  // se_step_plain_r{1,2,4}_nt{0,1}
-    std::vector<std::pair<se::LdsVariant, hipFunction_t>> lds;
-    int device = 0;
-    ~se_module() {
-        if (mod) {
-            int prev;
-            hipGetDevice(&prev);
-            hipSetDevice(device);
-            hipFree(table);
-            hipModuleUnload(mod);
-            hipSetDevice(prev);
-        }
-    }
-};
-
-struct se_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::mutex mu;
-    std::map<uint64_t, std::pair<std::string, std::shared_ptr<se_module>>> cache;
-};
-
-struct se_ruleset {
-    se_ctx* ctx = nullptr;
-    se::ParsingResult pr;
-    std::shared_ptr<se_module> mod;
-    int nmat = 0;
-};
-
-struct se_world {
-    se_ctx* ctx = nullptr;
-    const se_ruleset* rs = nullptr;
-    std::shared_ptr<se_module> mod;
-    uint32_t W = 0, H = 0, pitch = 0;
-    uint32_t* buf[2] = {nullptr, nullptr};
-    int cur = 0;
-    int32_t frame = 0;
-    std::vector<se_modification> pending;
-    void* d_mods = nullptr;
-    hipStream_t stream = nullptr;
-    int plain_variant = 3;  // index into se_module::plain_v (default: 1 row pair, non-temporal stores)
-    int lds_variant = -1;   // index into se_module::lds, -1 = default choice
-};
-
 namespace {
-
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        hipGetDevice(&prev);
-        if (prev != dev) hipSetDevice(dev);
-    }
-    ~DeviceGuard() {
-        int now;
-        hipGetDevice(&now);
-        if (prev >= 0 && now != prev) hipSetDevice(prev);
-    }
-};
 
 int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
                 std::shared_ptr<se_module>& out) {
@@ -241,7 +159,12 @@ int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
         unsigned bytes = (unsigned)(((v.tw + 2 * v.hb) * (v.th + 2 * v.hb) + 15) & ~15) +
                          (unsigned)(m->table_entries * 4);
         if ((int)bytes > max_lds) continue;  // does not fit this device's LDS
+        hipFunction_t fb;
+        std::string bname = se::lds_kernel_name(v);
+        bname.replace(0, std::strlen("se_step_lds_"), "se_batch_lds_");
+        HIPCHECK(hipModuleGetFunction(&fb, m->mod, bname.c_str()));
         m->lds.push_back({v, f});
+        m->batch.push_back(fb);
         m->lds_bytes.push_back(bytes);
     }
     if (m->lds.empty()) return fail(SE_ERR_JIT, "no LDS kernel variant fits the device's LDS");
@@ -284,8 +207,10 @@ int launch_fill(se_world* w, uint32_t* g, uint32_t v) {
     return launch(w->mod->fill, blocks, w->stream, g, n, v);
 }
 
+}  // namespace
+
 // One Simulation::run with the plain kernel.
-int plain_step(se_world* w) {
+int se_rt::plain_step(se_world* w) {
     w->frame += 1;
     uint32_t* in = w->buf[w->cur];
     uint32_t* out = w->buf[w->cur ^ 1];
@@ -314,6 +239,9 @@ int plain_step(se_world* w) {
     w->cur ^= 1;
     return SE_OK;
 }
+
+namespace {
+using se_rt::plain_step;
 
 // LDS instantiation for a launch of up to `sub` sub-steps (0: the variant's halo):
 // the world's tuned choice if set, else the 192x192 / halo 8 / 512-thread tile
@@ -356,6 +284,9 @@ int blocked_steps(se_world* w, uint32_t nsteps, uint32_t sub) {
     auto* v = pick_lds(w, sub);
     if (!v) return fail(SE_ERR_ARG, "no LDS variant supports " + std::to_string(sub) + " sub-steps");
     uint32_t k = sub ? sub : (uint32_t)v->first.hb;
+    // a world inside one tile keeps its WALL border in LDS: no halo limit on k
+    const bool single = w->W <= (uint32_t)v->first.tw && w->H <= (uint32_t)v->first.th;
+    if (single && sub == 0) k = 1u << 20;
     uint32_t done = 0;
     while (done < nsteps) {
         if (w->frame + 1 == 1) {  // the clearing frame stays a step of its own
@@ -654,23 +585,18 @@ int se_world_fill_seeded(se_world* w, uint64_t seed, const uint32_t* ids, const 
                          uint32_t stone_rows, uint32_t stone_id) {
     if (!w || !ids || !thresholds || n < 1 || n > 64) return fail(SE_ERR_ARG, "bad fill arguments");
     DeviceGuard dg(w->ctx->device);
-    void* d = nullptr;
-    HIPCHECK(hipMalloc(&d, 2 * 64 * sizeof(uint32_t)));
+    // ids and thresholds are staged in the world's modification buffer (8 KiB)
+    void* d = w->d_mods;
     uint32_t host[128] = {0};
     std::memcpy(host, ids, n * 4);
     std::memcpy(host + 64, thresholds, n * 4);
-    hipError_t e = hipMemcpy(d, host, sizeof host, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        hipFree(d);
-        return fail(SE_ERR_HIP, hipGetErrorString(e));
-    }
+    HIPCHECK(hipMemcpyAsync(d, host, sizeof host, hipMemcpyHostToDevice, w->stream));
     size_t cells = (size_t)w->W * w->H;
     unsigned blocks = (unsigned)std::min<size_t>((cells + 255) / 256, 1u << 16);
     int rc = launch(w->mod->fill_seeded, blocks, w->stream, w->buf[w->cur], (int)w->W, (int)w->H, (int)w->pitch,
                     (unsigned long long)seed, (const uint32_t*)d, (const uint32_t*)d + 64, n, (int)stone_rows,
                     stone_id);
     hipError_t e2 = hipStreamSynchronize(w->stream);
-    hipFree(d);
     if (rc) return rc;
     if (e2 != hipSuccess) return fail(SE_ERR_HIP, std::string("fill sync: ") + hipGetErrorString(e2));
     return SE_OK;
