@@ -32,6 +32,7 @@ typedef struct se_ctx se_ctx;
 typedef struct se_ruleset se_ruleset;
 typedef struct se_world se_world;
 typedef struct se_batch se_batch;
+typedef struct se_serve se_serve;
 
 /* == SimModification, sandengine-core/src/simulation.rs:46-56 (32 bytes).
  * mod_shape: 0 circle, 1 square (MODSHAPE_*, simulation.rs:41-42). */
@@ -161,6 +162,61 @@ int se_batch_run(se_batch* b, int mode, int32_t lds_variant, uint32_t k, int gra
  * snapshotted and restored) and keeps the fastest for later runs. */
 int se_batch_autotune(se_batch* b, uint32_t rounds, int32_t* best_variant, uint32_t* best_k,
                       float* best_ns_per_kcell);
+
+/* ---------------------------------------------------------------- serving loop
+ * No reference counterpart (the reference steps one interactive world per
+ * frame, sandengine-core/src/lib.rs:57-68).  Requests arrive by a seeded
+ * Poisson process; each is a square world with a rule set, a seeded i.i.d.
+ * fill, a step count and circle modifications (brushes of a random material,
+ * explosions of EMPTY) at given steps.  Admission is FIFO by HBM budget (one
+ * arena, no allocation per request); admitted worlds advance together in rounds
+ * (continuous batching) and each result equals the world stepped alone. */
+typedef struct {
+    double rate;              /* requests per second */
+    double duration_s;        /* arrival window */
+    uint64_t seed;
+    uint32_t side_min, side_max;   /* side log-uniform, rounded to even */
+    uint32_t steps_min, steps_max; /* steps log-uniform */
+    uint32_t mods_max;        /* modifications per request uniform in [0, mods_max] */
+    uint32_t quantum;         /* steps per round for multi-tile worlds (capped at the tile halo); 0 = 8 */
+    uint32_t quantum_small;   /* steps per round for worlds inside one LDS tile; 0 = 64 */
+    int kernel;               /* 1 batched LDS kernel, 0 plain kernel looped per world */
+    int max_inflight;         /* rounds queued on the device at once; 0 = 2 */
+    double drain_timeout_s;   /* stop this long after the last arrival (0 = run to completion) */
+    size_t hbm_budget;        /* bytes of world state admitted at once */
+    uint32_t verify_every;    /* keep a hash of every n-th request's final grid (0 = none) */
+} se_serve_config;
+
+typedef struct {
+    uint64_t requests, completed, rejected, unfinished, rounds;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* arrival -> completion, completed requests */
+    double wall_s, cell_updates, gcups;     /* completed work over the wall time */
+    double offered_gcups;                   /* arriving work per second of the arrival window */
+    size_t peak_bytes;                      /* arena high-water mark */
+    double processed_gcups;                 /* all steps run (also of unfinished requests) over wall */
+    double p50_ms_censored, p99_ms_censored;/* over every arrived request, unfinished ones at the end time */
+} se_serve_stats;
+
+typedef struct {
+    double arrival_s, done_s;
+    uint32_t side, steps, ruleset, nmods;
+    uint64_t fill_seed;
+    int state;    /* 0 waiting, 1 admitted, 2 done, 3 rejected (larger than the budget) */
+    int verified; /* hash holds FNV-1a 64 of the final grid (row-major u32) */
+    uint64_t hash;
+} se_serve_request;
+
+int se_serve_create(se_ctx* ctx, const se_ruleset* const* rulesets, int n_rulesets, const se_serve_config* cfg,
+                    se_serve** out);
+void se_serve_destroy(se_serve* s);
+/* The i.i.d. fill of requests that use rule set `rs` (see se_world_fill_seeded). */
+int se_serve_set_fill(se_serve* s, int rs, const uint32_t* ids, const uint32_t* thresholds, int n);
+int se_serve_run(se_serve* s, se_serve_stats* stats);
+uint64_t se_serve_request_count(const se_serve* s);
+int se_serve_request_info(const se_serve* s, uint64_t i, se_serve_request* out);
+int se_serve_request_mod(const se_serve* s, uint64_t i, uint32_t j, uint32_t* step, se_modification* m);
+/* FNV-1a 64 of a byte range (the hash se_serve_request.hash uses). */
+uint64_t se_fnv1a64(const void* data, size_t bytes);
 
 #ifdef __cplusplus
 }

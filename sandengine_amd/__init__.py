@@ -105,6 +105,16 @@ def lib():
         L.se_world_step_timed.argtypes = [c_p, c_u32, ctypes.c_int, c_u32, ctypes.POINTER(ctypes.c_float)]
         L.se_world_set_variant.argtypes = [c_p, c_i32, c_i32]
         L.se_world_copy_timed.argtypes = [c_p, c_u32, ctypes.POINTER(ctypes.c_float)]
+        L.se_serve_create.argtypes = [c_p, ctypes.POINTER(c_p), ctypes.c_int, c_p, PP]
+        L.se_serve_destroy.argtypes = [c_p]
+        L.se_serve_set_fill.argtypes = [c_p, ctypes.c_int, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32), ctypes.c_int]
+        L.se_serve_run.argtypes = [c_p, c_p]
+        L.se_serve_request_count.argtypes = [c_p]
+        L.se_serve_request_count.restype = ctypes.c_uint64
+        L.se_serve_request_info.argtypes = [c_p, ctypes.c_uint64, c_p]
+        L.se_serve_request_mod.argtypes = [c_p, ctypes.c_uint64, c_u32, ctypes.POINTER(c_u32), c_p]
+        L.se_fnv1a64.argtypes = [c_p, c_sz]
+        L.se_fnv1a64.restype = ctypes.c_uint64
         L.se_batch_create.argtypes = [c_p, PP]
         L.se_batch_destroy.argtypes = [c_p]
         L.se_batch_add.argtypes = [c_p, c_p, c_u32]
@@ -364,10 +374,7 @@ class World:
         """Device-side seeded fill; percents are integers summing to 100."""
         if sum(percents) != 100 or len(ids) != len(percents):
             raise ValueError("percents must sum to 100, one per id")
-        thr, cum = [], 0
-        for p in percents:
-            cum += p
-            thr.append(min((cum << 32) // 100, 0xFFFFFFFF))
+        thr = percent_thresholds(percents)
         a = (ctypes.c_uint32 * len(ids))(*ids)
         t = (ctypes.c_uint32 * len(ids))(*thr)
         _check(lib().se_world_fill_seeded(self._p, seed, a, t, len(ids), stone_rows, stone_id))
@@ -375,6 +382,98 @@ class World:
     def close(self):
         if getattr(self, "_p", None) and self._p.value:
             lib().se_world_destroy(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("rate", ctypes.c_double), ("duration_s", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("side_min", ctypes.c_uint32), ("side_max", ctypes.c_uint32), ("steps_min", ctypes.c_uint32),
+                ("steps_max", ctypes.c_uint32), ("mods_max", ctypes.c_uint32), ("quantum", ctypes.c_uint32),
+                ("quantum_small", ctypes.c_uint32), ("kernel", ctypes.c_int), ("max_inflight", ctypes.c_int),
+                ("drain_timeout_s", ctypes.c_double), ("hbm_budget", ctypes.c_size_t),
+                ("verify_every", ctypes.c_uint32)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+                ("unfinished", ctypes.c_uint64), ("rounds", ctypes.c_uint64), ("p50_ms", ctypes.c_double),
+                ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("wall_s", ctypes.c_double), ("cell_updates", ctypes.c_double), ("gcups", ctypes.c_double),
+                ("offered_gcups", ctypes.c_double), ("peak_bytes", ctypes.c_size_t),
+                ("processed_gcups", ctypes.c_double), ("p50_ms_censored", ctypes.c_double),
+                ("p99_ms_censored", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServeRequest(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("done_s", ctypes.c_double), ("side", ctypes.c_uint32),
+                ("steps", ctypes.c_uint32), ("ruleset", ctypes.c_uint32), ("nmods", ctypes.c_uint32),
+                ("fill_seed", ctypes.c_uint64), ("state", ctypes.c_int), ("verified", ctypes.c_int),
+                ("hash", ctypes.c_uint64)]
+
+
+def fnv1a64(a: np.ndarray) -> int:
+    b = np.ascontiguousarray(a)
+    return lib().se_fnv1a64(b.ctypes.data_as(ctypes.c_void_p), b.nbytes)
+
+
+def percent_thresholds(percents: Sequence[int]) -> List[int]:
+    """Cumulative integer percents -> u32 thresholds floor(cum * 2^32 / 100)."""
+    thr, cum = [], 0
+    for p in percents:
+        cum += p
+        thr.append(min((cum << 32) // 100, 0xFFFFFFFF))
+    return thr
+
+
+class Serve:
+    """Serving loop with continuous batching under an HBM budget (se_serve)."""
+
+    def __init__(self, ctx: Context, rulesets: Sequence[RuleSet], fills, **cfg):
+        c = ServeConfig()
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        arr = (ctypes.c_void_p * len(rulesets))(*[r._p.value for r in rulesets])
+        p = ctypes.c_void_p()
+        _check(lib().se_serve_create(ctx._p, arr, len(rulesets), ctypes.byref(c), ctypes.byref(p)))
+        self._p, self.ctx, self.rulesets, self.cfg = p, ctx, list(rulesets), c
+        for i, (ids, pct) in enumerate(fills):
+            a = (ctypes.c_uint32 * len(ids))(*ids)
+            t = (ctypes.c_uint32 * len(ids))(*percent_thresholds(pct))
+            _check(lib().se_serve_set_fill(self._p, i, a, t, len(ids)))
+
+    def run(self) -> dict:
+        st = ServeStats()
+        _check(lib().se_serve_run(self._p, ctypes.byref(st)))
+        return st.as_dict()
+
+    def __len__(self):
+        return lib().se_serve_request_count(self._p)
+
+    def request(self, i: int) -> dict:
+        r = ServeRequest()
+        _check(lib().se_serve_request_info(self._p, i, ctypes.byref(r)))
+        d = {k: getattr(r, k) for k, _ in r._fields_}
+        mods = []
+        for j in range(r.nmods):
+            step = ctypes.c_uint32()
+            m = SeModification()
+            _check(lib().se_serve_request_mod(self._p, i, j, ctypes.byref(step), ctypes.byref(m)))
+            mods.append((step.value, [m.position[0], m.position[1], m.mod_shape, m.mod_size, m.mod_matID]))
+        d["mods"] = mods
+        return d
+
+    def close(self):
+        if getattr(self, "_p", None) and self._p.value:
+            lib().se_serve_destroy(self._p)
             self._p = ctypes.c_void_p()
 
     def __del__(self):

@@ -30,8 +30,11 @@ struct BWorld {
     int frame0;
     int steps;
     int cur0;
+    const void* mods;
+    int nmods;
+    int pad;
 };
-static_assert(sizeof(BWorld) == 48, "se_bworld layout");
+static_assert(sizeof(BWorld) == 64, "se_bworld layout");
 
 struct DevArray {
     void* p = nullptr;
@@ -86,6 +89,9 @@ int upload_list(const std::vector<std::pair<se_world*, uint32_t>>& list, const s
         b.frame0 = w->frame + 1;
         b.steps = (int)it.second;
         b.cur0 = w->cur;
+        b.mods = nullptr;
+        b.nmods = 0;
+        b.pad = 0;
         recs.push_back(b);
         pre.push_back(pre.back() + b.ntiles);
     }

@@ -64,7 +64,7 @@ struct se_module {
     uint32_t* table = nullptr;       // transition table (SE_TBL), device memory
     size_t table_entries = 0;        // 0 when the rule set runs the generated chain
     std::vector<unsigned> lds_bytes; // dynamic LDS per LDS variant
-    hipFunction_t plain = nullptr, fill = nullptr, fill_seeded = nullptr, copy = nullptr;
+    hipFunction_t plain = nullptr, fill = nullptr, fill_seeded = nullptr, copy = nullptr, fill_spec = nullptr;
     hipFunction_t plain_v[6] = {};  // se_step_plain_r{1,2,4}_nt{0,1}
     std::vector<std::pair<se::LdsVariant, hipFunction_t>> lds;
     std::vector<hipFunction_t> batch;  // se_batch_lds_* parallel to `lds`
@@ -106,6 +106,7 @@ struct se_world {
     std::vector<se_modification> pending;
     void* d_mods = nullptr;
     hipStream_t stream = nullptr;
+    bool owns = true;       // buffers from hipMalloc (false: carved from a serving arena)
     int plain_variant = 3;  // index into se_module::plain_v (default: 1 row pair, non-temporal stores)
     int lds_variant = -1;   // index into se_module::lds, -1 = default choice
 };

@@ -469,6 +469,9 @@ struct se_bworld {
     int frame0;      // frame of the world's first step in this run
     int steps;       // steps the world advances in this run
     int cur0;
+    const se_mod* mods;  // modifications for the world's first step in this run
+    int nmods;
+    int pad;
 };
 
 // Index of the world owning item `item`: prefix[w] <= item < prefix[w+1].
@@ -510,7 +513,8 @@ __device__ __forceinline__ int se_owner(const int* __restrict__ prefix, int n, i
             const int nsub = min(k, B.steps - done);                                                          \
             const int cur = (B.cur0 + round) & 1;                                                             \
             se_lds_body<TW, TH, HB, NT>(B.buf[cur], B.buf[cur ^ 1], B.W, B.H, B.pitch, B.frame0 + done, nsub,  \
-                                        B.tiles_x, item - prefix[w], nullptr, 0, lt, se_lds);                 \
+                                        B.tiles_x, item - prefix[w], done == 0 ? B.mods : nullptr,            \
+                                        done == 0 ? B.nmods : 0, lt, se_lds);                                 \
         }                                                                                                     \
     }
 
@@ -524,6 +528,30 @@ __device__ __forceinline__ u64 se_splitmix64(u64 z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
+}
+
+struct se_fillspec {  // fill parameters passed by value
+    int n, stone_rows;
+    u32 stone_id, pad;
+    u32 ids[16], thr[16];
+};
+
+extern "C" __global__ __launch_bounds__(256) void se_fill_spec(u32* __restrict__ g, int W, int H, int pitch,
+                                                               u64 seed, se_fillspec sp) {
+    const size_t total = (size_t)W * (size_t)H;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        const size_t y = i / (size_t)W, x = i - y * (size_t)W;
+        u32 v;
+        if ((long)y >= (long)H - sp.stone_rows) {
+            v = sp.stone_id;
+        } else {
+            const u32 r = (u32)(se_splitmix64((seed << 40) ^ (u64)i) >> 32);
+            v = sp.ids[sp.n - 1];
+            for (int k = 0; k < sp.n - 1; k++)
+                if (r < sp.thr[k]) { v = sp.ids[k]; break; }
+        }
+        g[y * (size_t)pitch + x] = v;
+    }
 }
 
 extern "C" __global__ __launch_bounds__(256) void se_fill_seeded(u32* __restrict__ g, int W, int H, int pitch,

@@ -140,6 +140,7 @@ int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
     HIPCHECK(hipModuleGetFunction(&m->fill, m->mod, "se_fill"));
     HIPCHECK(hipModuleGetFunction(&m->fill_seeded, m->mod, "se_fill_seeded"));
     HIPCHECK(hipModuleGetFunction(&m->copy, m->mod, "se_copy"));
+    HIPCHECK(hipModuleGetFunction(&m->fill_spec, m->mod, "se_fill_spec"));
     int max_lds = 65536;
     hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
     if (info.table) {
@@ -474,9 +475,11 @@ void se_world_destroy(se_world* w) {
     {
         DeviceGuard dg(w->ctx->device);
         hipStreamSynchronize(w->stream);
-        hipFree(w->buf[0]);
-        hipFree(w->buf[1]);
-        hipFree(w->d_mods);
+        if (w->owns) {
+            hipFree(w->buf[0]);
+            hipFree(w->buf[1]);
+            hipFree(w->d_mods);
+        }
     }
     delete w;
 }
