@@ -305,14 +305,19 @@ def main():
     wc.close()
     copy_gbs = 8.0 * cells / (copy_ms / 1e3) / 1e9
     log(f"copy reference: {copy_ms:.3f} ms per 2x{4 * cells / 2**30:.2f} GiB -> {copy_gbs:.0f} GB/s")
+    lds_kernel = None
     for mode in args.paths.split(","):
         w = make_world(S, ctx, rs, args.workload)
+        if mode == "blocked":
+            tw, th, c = w.variant_info(1, w.variant_info(2, args.sub)[0])
+            lds_kernel = f"se_step_lds_{tw}x{th}_h{c // 1000}_t{c % 1000}"
         wall, gpu_ms = time_path(S, w, mode, args.steps, args.warmup, args.sub)
         launches = args.steps if mode == "plain" else -(-args.steps // args.sub)
         avg_launch_ms = gpu_ms / launches
         r = dict(gcups=cells * args.steps / wall / 1e9, gcups_gpu=cells * args.steps / (gpu_ms / 1e3) / 1e9,
                  ms_per_step=wall * 1e3 / args.steps, avg_launch_ms=avg_launch_ms, launches=launches,
-                 steps_per_launch=1 if mode == "plain" else args.sub)
+                 steps_per_launch=1 if mode == "plain" else args.sub,
+                 kernel="se_step_plain_r1_nt1" if mode == "plain" else lds_kernel)
         r["achieved_gbs"] = 8.0 * cells / (avg_launch_ms / 1e3) / 1e9
         r["roofline_frac"] = r["achieved_gbs"] / HBM_PEAK_GBS
         r["frac_of_copy"] = r["achieved_gbs"] / copy_gbs
@@ -339,7 +344,7 @@ def main():
         w.close()
     best = max(results, key=lambda m: results[m]["gcups"])
     R = results[best]
-    kernel = "se_step_plain_r1_nt1" if best == "plain" else "se_step_lds_192x192_h8_t512"
+    kernel = "se_step_plain_r1_nt1" if best == "plain" else lds_kernel
     traffic = load_traffic(kernel)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload)
     out = {

@@ -777,6 +777,21 @@ std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info) {
          "unsigned int frame) {\n";
     for (int l = 0; l < 4; l++) o << "    if (l == " << l << ") return " << hash_expr(l) << ";\n";
     o << "    return 0u;\n}\n";
+    // the same hash split into a column term gx*ka, a row term gy*kb and a frame
+    // term frame*kf (all mod 2^32), for kernels that keep the terms across blocks
+    {
+        const char* nm[3] = {"ka", "kb", "kf"};
+        const uint32_t base[3] = {461u, 2131u, 4541161u};
+        for (int k = 0; k < 3; k++) {
+            o << "__device__ __forceinline__ unsigned int se_hash_" << nm[k] << "(int l) {\n";
+            for (int l = 0; l < 4; l++) o << "    if (l == " << l << ") return " << (uint32_t)(base[k] * mult[l]) << "u;\n";
+            o << "    return 0u;\n}\n";
+        }
+        o << "__device__ __forceinline__ bool se_hash_used(int l) {\n    return l == 0";
+        for (int l = 1; l < 4; l++)
+            if (em.lanes_used.count(l)) o << " || l == " << l;
+        o << ";\n}\n";
+    }
     o << "__device__ __forceinline__ void se_block(unsigned int& r0, unsigned int& r1, unsigned int& r2, "
          "unsigned int& r3, unsigned int ox, unsigned int oy, unsigned int frame) {\n";
     o << "    if ((r0 | r1 | r2 | r3) == 0u) return;\n";

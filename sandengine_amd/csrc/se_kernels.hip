@@ -325,6 +325,106 @@ __device__ __forceinline__ void se_block_range(int s, int par, int g0, int L, in
     cnt = b >= a ? (b - a) / 2 + 1 : 0;
 }
 
+#if SE_TBL
+// Tuple index c0 + N*c1 + N^2*c2 + N^3*c3 of a packed block t = c0 | c1<<8 |
+// c2<<16 | c3<<24 (ids < N): byte dot products with constant weights.
+__device__ __forceinline__ u32 se_tuple(u32 t) {
+#if SE_NMAT <= 6
+    return __builtin_amdgcn_udot4(t, 1u | (SE_NMAT << 8) | ((SE_NMAT * SE_NMAT) << 16) |
+                                         ((SE_NMAT * SE_NMAT * SE_NMAT) << 24), 0u, false);
+#else
+    const u32 lo = __builtin_amdgcn_udot4(t, 1u | (SE_NMAT << 8), 0u, false);
+    const u32 hi = __builtin_amdgcn_udot4(t, (1u << 16) | (SE_NMAT << 24), 0u, false);
+    return lo + __umul24(hi, SE_NMAT * SE_NMAT);
+#endif
+}
+#endif
+
+// One Margolus sub-step over the block region (fx + 2*bx, fy + 2*by), bx < nbx,
+// by < nby, of an LDS tile with row length LW, without modifications.  Lanes of
+// a wavefront take consecutive blocks of one block row (64 per pass), the
+// wavefronts of the workgroup take every NW-th block row, so each lane keeps its
+// column terms (LDS column, the hash's x term) and the row terms are uniform.
+// With x offset 0 a block's cell pairs are 2-byte aligned and move as 16-bit
+// LDS accesses; with x offset 1 they move as bytes (misaligned 16-bit LDS
+// accesses are much slower).  In a tile that reaches past the world (`edge`), cells
+// outside the world stay WALL: the result bytes of those cells are replaced by
+// WALL before the store.
+template <int LW, int NT>
+__device__ __forceinline__ void se_lds_substep(unsigned char* lds, const u32* __restrict__ ltbl, int fx, int nbx,
+                                               int fy, int nby, int gx0, int gy0, int W, int H, u32 f, bool edge) {
+    constexpr int NW = NT / 64;
+    const bool odd = (fx & 1) != 0;  // block origins share the parity of the x offset
+    const int lane = (int)(threadIdx.x & 63);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    u32 kf[4];
+#pragma unroll
+    for (int l = 0; l < 4; l++) kf[l] = f * se_hash_kf(l);
+    for (int bx0 = 0; bx0 < nbx; bx0 += 64) {
+        const int bx = bx0 + lane;
+        if (bx >= nbx) break;
+        const int lx = fx + 2 * bx;
+        const int gx = gx0 + lx;
+        u32 xa[4];
+#pragma unroll
+        for (int l = 0; l < 4; l++) xa[l] = (u32)gx * se_hash_ka(l);
+        u32 colkeep = 0xffffffffu;
+        if (edge)
+            colkeep = ((gx >= 0 && gx < W) ? 0x00ff00ffu : 0u) | ((gx + 1 >= 0 && gx + 1 < W) ? 0xff00ff00u : 0u);
+        for (int by = wv; by < nby; by += NW) {
+            const int ly = fy + 2 * by;
+            const int gy = gy0 + ly;
+            unsigned char* p0 = lds + ly * LW + lx;
+            u32 t;
+            if (odd)
+                t = se_pack4(p0[0], p0[1], p0[LW], p0[LW + 1]);
+            else
+                t = (u32)*reinterpret_cast<const unsigned short*>(p0) |
+                    ((u32)*reinterpret_cast<const unsigned short*>(p0 + LW) << 16);
+            u32 h[4];
+#pragma unroll
+            for (int l = 0; l < 4; l++) h[l] = 0u;
+            u32 e;
+#if SE_TBL
+            const u32 base = se_tuple(t) * SE_TBL_R;
+            e = ltbl[base];
+            if (!(e >> 31)) {  // the result depends on the block's hash classes
+#pragma unroll
+                for (int l = 0; l < 4; l++)
+                    if (se_hash_used(l)) h[l] = se_lowbias(xa[l] + (u32)gy * se_hash_kb(l) + kf[l]);
+                e = ltbl[base + se_rclass(h[0], h[1], h[2], h[3])];
+            }
+            e &= 0x7fffffffu;
+#else
+            e = t;
+            if (t != 0u) {
+#pragma unroll
+                for (int l = 0; l < 4; l++)
+                    if (se_hash_used(l)) h[l] = se_lowbias(xa[l] + (u32)gy * se_hash_kb(l) + kf[l]);
+                u32 c0 = t & 0xffu, c1 = (t >> 8) & 0xffu, c2 = (t >> 16) & 0xffu, c3 = t >> 24;
+                se_block_h(c0, c1, c2, c3, h[0], h[1], h[2], h[3]);
+                e = se_pack4(c0, c1, c2, c3);
+            }
+#endif
+            if (edge) {
+                const u32 rowkeep =
+                    ((gy >= 0 && gy < H) ? 0x0000ffffu : 0u) | ((gy + 1 >= 0 && gy + 1 < H) ? 0xffff0000u : 0u);
+                const u32 keep = colkeep & rowkeep;
+                e = (e & keep) | (0x02020202u & ~keep);
+            }
+            if (odd) {
+                p0[0] = (unsigned char)e;
+                p0[1] = (unsigned char)(e >> 8);
+                p0[LW] = (unsigned char)(e >> 16);
+                p0[LW + 1] = (unsigned char)(e >> 24);
+            } else {
+                *reinterpret_cast<unsigned short*>(p0) = (unsigned short)e;
+                *reinterpret_cast<unsigned short*>(p0 + LW) = (unsigned short)(e >> 16);
+            }
+        }
+    }
+}
+
 template <int TW, int TH, int HB, int NT>
 __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __restrict__ out, int W, int H,
                                             int pitch, int frame0, int nsub, int tiles_x, int tile,
@@ -336,8 +436,29 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     const int gx0 = tx * TW - HB, gy0 = ty * TH - HB;
     const bool interior = gx0 >= 0 && gy0 >= 0 && gx0 + LW <= W && gy0 + LH <= H;
     u32* lds32 = reinterpret_cast<u32*>(lds);
-    // tile load: rows/quads that touch the world plus its one-cell WALL border
-    {
+    if (interior) {
+        // whole tile inside the world: every thread issues U quad loads before
+        // any of them is decoded into LDS, so a thread waits U loads at a time
+        constexpr int TOTAL = LQ * LH, U = 8;
+        for (int i0 = 0; i0 < TOTAL; i0 += U * NT) {
+            uint4 v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * NT + tid;
+                if (i < TOTAL) {
+                    const int ly = i / LQ, lq = i - ly * LQ;
+                    v[u] = *reinterpret_cast<const uint4*>(in + (size_t)(gy0 + ly) * pitch + gx0 + 4 * lq);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * NT + tid;
+                if (i < TOTAL)
+                    lds32[i] = se_pack4(se_decode(v[u].x), se_decode(v[u].y), se_decode(v[u].z), se_decode(v[u].w));
+            }
+        }
+    } else {
+        // tile load: rows/quads that touch the world plus its one-cell WALL border
         const int ly_lo = max(0, -1 - gy0), ly_hi = min(LH, H + 1 - gy0);
         const int lq_lo = max(0, (-1 - gx0) >> 2), lq_hi = min(LQ, (W + 1 - gx0 + 3) >> 2);
         const int nq = max(0, lq_hi - lq_lo), nr = max(0, ly_hi - ly_lo);
@@ -371,7 +492,9 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
         se_block_range(s, oy, gy0, LH, H, fy, nby);
         const int nb = nbx * nby;
         const bool use_mods = s == 0 && nmods > 0;
-        if (nb > 0) {
+        if (!use_mods) {
+            se_lds_substep<LW, NT>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f, !interior);
+        } else if (nb > 0) {
             const int dby = NT / nbx, dbx = NT - dby * nbx;
             int by = tid / nbx, bx = tid - by * nbx;
             for (int b = tid; b < nb; b += NT) {
@@ -431,6 +554,29 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     }
     // interior store: quads from LDS, 16-byte global stores where inside the world
     constexpr int TQ = TW / 4;
+    if (gx0 + HB + TW <= W && gy0 + HB + TH <= H) {
+        constexpr int TOTAL = TQ * TH, U = 4;
+        for (int i0 = 0; i0 < TOTAL; i0 += U * NT) {
+            u32 p[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * NT + tid;
+                const int ly = i / TQ, lq = i - ly * TQ;
+                p[u] = i < TOTAL ? lds32[((ly + HB) * LW + HB) / 4 + lq] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * NT + tid;
+                if (i < TOTAL) {
+                    const int ly = i / TQ, lq = i - ly * TQ;
+                    const uint4 v = make_uint4(p[u] & 0xffu, (p[u] >> 8) & 0xffu, (p[u] >> 16) & 0xffu, p[u] >> 24);
+                    se_store_quad<true>(out + (size_t)(gy0 + HB + ly) * pitch + gx0 + HB + 4 * lq, v);
+                }
+            }
+        }
+        __syncthreads();  // LDS is reused by the next item of a persistent workgroup
+        return;
+    }
     const int sy_hi = max(0, min(TH, H - (gy0 + HB))), sq_hi = max(0, min(TQ, (W - (gx0 + HB) + 3) >> 2));
     for (int i = tid; i < sq_hi * sy_hi; i += NT) {
         const int ly = i / sq_hi, lq = i - ly * sq_hi;
@@ -453,7 +599,24 @@ __device__ __forceinline__ const u32* se_stage_table(const u32* __restrict__ tbl
                                                      int nt) {
 #if SE_TBL
     u32* ltbl = reinterpret_cast<u32*>(lds + ((lds_tile + 15) & ~15));
-    for (int i = threadIdx.x; i < (int)SE_TBL_ENTRIES; i += nt) ltbl[i] = tbl[i];
+    // 16-byte loads, U per thread in flight before their LDS stores
+    constexpr int Q = (int)SE_TBL_ENTRIES / 4, U = 4;
+    const uint4* src = reinterpret_cast<const uint4*>(tbl);
+    uint4* dst = reinterpret_cast<uint4*>(ltbl);
+    for (int i0 = 0; i0 < Q; i0 += U * nt) {
+        uint4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = i0 + u * nt + (int)threadIdx.x;
+            if (i < Q) v[u] = src[i];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = i0 + u * nt + (int)threadIdx.x;
+            if (i < Q) dst[i] = v[u];
+        }
+    }
+    for (int i = 4 * Q + (int)threadIdx.x; i < (int)SE_TBL_ENTRIES; i += nt) ltbl[i] = tbl[i];
     __syncthreads();
     return ltbl;
 #else
@@ -485,6 +648,16 @@ __device__ __forceinline__ int se_owner(const int* __restrict__ prefix, int n, i
     return lo;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Workgroup b takes tile (b % 8) * q + b / 8 (q = n / 8; the last n % 8 keep
+// their index), so each XCD walks one contiguous run of tiles and neighbouring
+// tiles, which share halo lines, are fetched through the same L2.
+__device__ __forceinline__ int se_xcd_tile(int b, int n) {
+    const int q = n >> 3;
+    if (b >= (q << 3)) return b;
+    return (b & 7) * q + (b >> 3);
+}
+
 // dynamic LDS: tile bytes rounded to 16, then the table (SE_TBL)
 //  se_step_lds_*  : one world, one workgroup per tile.
 //  se_batch_lds_* : persistent over many worlds: workgroups stride over the
@@ -497,7 +670,8 @@ __device__ __forceinline__ int se_owner(const int* __restrict__ prefix, int n, i
         int tiles_x, const se_mod* __restrict__ mods, int nmods, const u32* __restrict__ tbl) {               \
         extern __shared__ __align__(16) unsigned char se_lds[];                                               \
         const u32* lt = se_stage_table(tbl, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT);                      \
-        se_lds_body<TW, TH, HB, NT>(in, out, W, H, pitch, frame0, nsub, tiles_x, (int)blockIdx.x, mods, nmods, \
+        se_lds_body<TW, TH, HB, NT>(in, out, W, H, pitch, frame0, nsub, tiles_x,                               \
+                                    se_xcd_tile((int)blockIdx.x, (int)gridDim.x), mods, nmods,                  \
                                     lt, se_lds);                                                              \
     }                                                                                                         \
     extern "C" __global__ __launch_bounds__(NT) void se_batch_lds_##TW##x##TH##_h##HB##_t##NT(               \

@@ -48,14 +48,16 @@ char* dupstr(const std::string& s) {
 
 const std::vector<se::LdsVariant>& lds_variants() {
     static const std::vector<se::LdsVariant> v = {
-        {128, 128, 8, 256}, {256, 128, 8, 512}, {128, 128, 16, 256}, {192, 192, 8, 512}, {256, 64, 8, 256},
-        {128, 64, 8, 256}};
+        {240, 112, 8, 512}, {240, 240, 8, 512}, {240, 176, 8, 512}, {112, 112, 8, 256}, {224, 224, 16, 512},
+        {192, 192, 8, 512}};
     return v;
 }
 
-// hiprtc options: gfx950, IEEE f32 with no contraction, and packed-f32 forms off
+// hiprtc options: gfx950, IEEE f32 with no contraction, packed-f32 forms off, and
+// no merging of byte LDS accesses into misaligned wider ones (slow on LDS)
 const char* kJitOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
-                          "-Wno-parentheses-equality", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+                          "-Wno-parentheses-equality", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                          "-Xclang", "-target-feature", "-Xclang", "-unaligned-access-mode"};
 
 int jit_compile(const std::string& src, std::vector<char>& code) {
     hiprtcProgram prog;
@@ -245,9 +247,11 @@ namespace {
 using se_rt::plain_step;
 
 // LDS instantiation for a launch of up to `sub` sub-steps (0: the variant's halo):
-// the world's tuned choice if set, else the 192x192 / halo 8 / 512-thread tile
-// (fastest measured on a 16384^2 world) when it allows `sub`, else the variant
-// with the smallest halo >= sub.
+// the world's tuned choice if set, else the default tile (fastest measured on a
+// 16384^2 world) when it allows `sub`, else the variant with the smallest halo
+// >= sub.
+constexpr se::LdsVariant kDefaultLds = {112, 112, 8, 256};
+
 const std::pair<se::LdsVariant, hipFunction_t>* pick_lds(se_world* w, uint32_t sub) {
     auto& L = w->mod->lds;
     if (w->lds_variant >= 0) {
@@ -255,7 +259,9 @@ const std::pair<se::LdsVariant, hipFunction_t>* pick_lds(se_world* w, uint32_t s
         return (sub == 0 || (uint32_t)p.first.hb >= sub) ? &p : nullptr;
     }
     for (auto& p : L)
-        if (p.first.tw == 192 && p.first.th == 192 && p.first.hb == 8 && (sub == 0 || sub <= 8)) return &p;
+        if (p.first.tw == kDefaultLds.tw && p.first.th == kDefaultLds.th && p.first.hb == kDefaultLds.hb &&
+            (sub == 0 || sub <= (uint32_t)kDefaultLds.hb))
+            return &p;
     const std::pair<se::LdsVariant, hipFunction_t>* best = nullptr;
     for (auto& p : L) {
         if (sub != 0 && (uint32_t)p.first.hb < sub) continue;
@@ -546,6 +552,14 @@ int se_variant_info(const se_world* w, int32_t kind, int32_t index, int32_t* a, 
         static const int kRows[3] = {1, 2, 4};
         if (a) *a = kRows[index % 3];
         if (b) *b = index / 3;
+        if (c) *c = 0;
+        return SE_OK;
+    }
+    if (kind == 2) {  // the LDS variant se_world_step_blocked(w, n, index) launches
+        auto* p = pick_lds(const_cast<se_world*>(w), (uint32_t)std::max(index, 0));
+        if (!p) return fail(SE_ERR_ARG, "no LDS variant supports " + std::to_string(index) + " sub-steps");
+        if (a) *a = (int32_t)(p - &w->mod->lds[0]);
+        if (b) *b = 0;
         if (c) *c = 0;
         return SE_OK;
     }
