@@ -119,7 +119,9 @@ int se_world_step_blocked(se_world* w, uint32_t nsteps, uint32_t sub);
  * store kind), lds_variant the LDS-tile instantiations (-1: default).
  * se_variant_info(kind 0 plain: a=rows, b=non-temporal; kind 1 lds: a=tile w,
  * b=tile h, c=max sub-steps * 1000 + threads per workgroup; kind 2: a=the LDS
- * variant se_world_step_blocked uses for `index` sub-steps per launch). */
+ * variant se_world_step_blocked uses for `index` sub-steps per launch; kind 3:
+ * a=workgroups of LDS variant `index` resident at once on the device (the
+ * serving loop's persistent grid), b=its dynamic LDS bytes). */
 int se_world_set_variant(se_world* w, int32_t plain_variant, int32_t lds_variant);
 int se_variant_info(const se_world* w, int32_t kind, int32_t index, int32_t* a, int32_t* b, int32_t* c);
 /* Params.frame (simulation.rs:78, 201). Negative frames are rejected. */

@@ -65,7 +65,7 @@ struct Plan {
     std::vector<std::pair<se_world*, uint32_t>> small, large;  // (world, steps), steps descending
     std::vector<int> n_active;                                  // per round, large list
     DevArray d_small, d_large, d_pre_small, d_pre_large;
-    unsigned grid = 0;
+    std::vector<int> pre_small, pre_large;                      // host copies of the tile prefix sums
 };
 
 bool fits_one_tile(const se_world* w, const se::LdsVariant& v) {
@@ -73,9 +73,9 @@ bool fits_one_tile(const se_world* w, const se::LdsVariant& v) {
 }
 
 int upload_list(const std::vector<std::pair<se_world*, uint32_t>>& list, const se::LdsVariant& v, DevArray& dw,
-                DevArray& dp) {
+                DevArray& dp, std::vector<int>& pre) {
     std::vector<BWorld> recs;
-    std::vector<int> pre(1, 0);
+    pre.assign(1, 0);
     for (auto& it : list) {
         se_world* w = it.first;
         BWorld b;
@@ -127,20 +127,16 @@ int make_plan(se_ctx* ctx, se_module* mod, int vi, uint32_t k, const std::vector
             P.n_active.push_back(n);
         }
     }
-    int rc = upload_list(P.small, v, P.d_small, P.d_pre_small);
-    if (!rc) rc = upload_list(P.large, v, P.d_large, P.d_pre_large);
-    if (rc) return rc;
-    int cus = 0, lds_cu = 0;
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
-    hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, ctx->device);
-    if (cus <= 0) cus = 256;
-    if (lds_cu <= 0) lds_cu = 160 * 1024;
-    int per_cu = std::max(1, std::min(lds_cu / (int)mod->lds_bytes[vi], 2048 / v.nt));
-    P.grid = (unsigned)(cus * per_cu);
-    return SE_OK;
+    int rc = upload_list(P.small, v, P.d_small, P.d_pre_small, P.pre_small);
+    if (!rc) rc = upload_list(P.large, v, P.d_large, P.d_pre_large, P.pre_large);
+    return rc;
 }
 
 // Enqueues a plan's launches: single-tile worlds on `s_small`, rounds on `s_large`.
+// One workgroup per work item (the kernel's item loop then runs once): workgroups
+// that start as others retire keep loads and compute of different items
+// overlapped, and the longest-first order makes the dispatch order a
+// longest-first schedule.
 int enqueue_plan(Plan& P, hipStream_t s_small, hipStream_t s_large) {
     const se::LdsVariant& v = P.mod->lds[P.vi].first;
     hipFunction_t f = P.mod->batch[P.vi];
@@ -148,12 +144,13 @@ int enqueue_plan(Plan& P, hipStream_t s_small, hipStream_t s_large) {
     const uint32_t* tbl = P.mod->table;
     if (!P.small.empty()) {
         int n = (int)P.small.size();
-        int rc = launch_ex(f, P.grid, (unsigned)v.nt, shmem, s_small, (const void*)P.d_small.p,
+        int rc = launch_ex(f, (unsigned)P.pre_small.back(), (unsigned)v.nt, shmem, s_small, (const void*)P.d_small.p,
                            (const int*)P.d_pre_small.p, n, 0, (int)(1 << 30), tbl);
         if (rc) return rc;
     }
     for (size_t r = 0; r < P.n_active.size(); r++) {
-        int rc = launch_ex(f, P.grid, (unsigned)v.nt, shmem, s_large, (const void*)P.d_large.p,
+        int rc = launch_ex(f, (unsigned)P.pre_large[P.n_active[r]], (unsigned)v.nt, shmem, s_large,
+                           (const void*)P.d_large.p,
                            (const int*)P.d_pre_large.p, P.n_active[r], (int)r, (int)P.k, tbl);
         if (rc) return rc;
     }

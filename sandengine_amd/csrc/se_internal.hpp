@@ -64,6 +64,7 @@ struct se_module {
     uint32_t* table = nullptr;       // transition table (SE_TBL), device memory
     size_t table_entries = 0;        // 0 when the rule set runs the generated chain
     std::vector<unsigned> lds_bytes; // dynamic LDS per LDS variant
+    std::vector<int> lds_grid;       // resident workgroups per LDS variant at full occupancy (multiple of 8)
     hipFunction_t plain = nullptr, fill = nullptr, fill_seeded = nullptr, copy = nullptr, fill_spec = nullptr;
     hipFunction_t plain_v[6] = {};  // se_step_plain_r{1,2,4}_nt{0,1}
     std::vector<std::pair<se::LdsVariant, hipFunction_t>> lds;

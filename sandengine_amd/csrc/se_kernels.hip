@@ -307,6 +307,15 @@ __device__ __forceinline__ void se_offset(u32 f, int& ox, int& oy) {
 
 __device__ __forceinline__ u32 se_pack4(u32 a, u32 b, u32 c, u32 d) { return a | (b << 8) | (c << 16) | (d << 24); }
 
+// Four cells from HBM as LDS bytes, ids >= SE_NMAT decoded to NULL (one range
+// test for the quad; the per-cell decode runs only where one is out of range).
+__device__ __forceinline__ u32 se_pack_quad(uint4 v) {
+    const u32 mx = max(max(v.x, v.y), max(v.z, v.w));
+    if (__builtin_expect(mx >= SE_NMAT, 0))
+        return se_pack4(se_decode(v.x), se_decode(v.y), se_decode(v.z), se_decode(v.w));
+    return se_pack4(v.x, v.y, v.z, v.w);
+}
+
 // Block-origin range along one axis at sub-step s: origins o (local) with parity
 // `par`, whose block touches the world (g0 + o in [-1, n-1]) and lies in the valid
 // region.  A side where the LDS tile cuts into the world loses one cell of
@@ -340,6 +349,79 @@ __device__ __forceinline__ u32 se_tuple(u32 t) {
 }
 #endif
 
+// A block's four cells from LDS as c0 | c1<<8 | c2<<16 | c3<<24: two 16-bit
+// accesses at even x, four byte accesses at odd x.
+template <int LW, bool ODD>
+__device__ __forceinline__ u32 se_blk_read(const unsigned char* p0) {
+    if (ODD) return se_pack4(p0[0], p0[1], p0[LW], p0[LW + 1]);
+    return (u32)*reinterpret_cast<const unsigned short*>(p0) |
+           ((u32)*reinterpret_cast<const unsigned short*>(p0 + LW) << 16);
+}
+
+template <int LW, bool ODD>
+__device__ __forceinline__ void se_blk_write(unsigned char* p0, u32 e) {
+    if (ODD) {
+        p0[0] = (unsigned char)e;
+        p0[1] = (unsigned char)(e >> 8);
+        p0[LW] = (unsigned char)(e >> 16);
+        p0[LW + 1] = (unsigned char)(e >> 24);
+    } else {
+        *reinterpret_cast<unsigned short*>(p0) = (unsigned short)e;
+        *reinterpret_cast<unsigned short*>(p0 + LW) = (unsigned short)(e >> 16);
+    }
+}
+
+// In an edge tile, the bytes of cells outside the world (columns by colkeep,
+// rows from gy) are replaced by WALL, which those cells hold in LDS.
+__device__ __forceinline__ u32 se_keep_wall(u32 e, u32 colkeep, int gy, int H) {
+    const u32 rowkeep = ((gy >= 0 && gy < H) ? 0x0000ffffu : 0u) | ((gy + 1 >= 0 && gy + 1 < H) ? 0xffff0000u : 0u);
+    const u32 keep = colkeep & rowkeep;
+    return (e & keep) | (0x02020202u & ~keep);
+}
+
+__device__ __forceinline__ void se_hash_lanes(u32* h, const u32* xa, const u32* kf, int gy) {
+#pragma unroll
+    for (int l = 0; l < 4; l++) h[l] = se_hash_used(l) ? se_lowbias(xa[l] + (u32)gy * se_hash_kb(l) + kf[l]) : 0u;
+}
+
+// Table path: new packed blocks for two packed blocks ta (world row gya) and tb
+// (row gyb).  Both class-0 entries are read at once; the hashes are computed
+// (for both) only where one of the two results depends on the hash classes.
+// xa[l] + gy * kb(l) + kf[l] is the hash input of lane l.
+__device__ __forceinline__ void se_blk_eval2(const u32* __restrict__ ltbl, u32 ta, u32 tb, const u32* xa,
+                                             const u32* kf, int gya, int gyb, u32& ea, u32& eb) {
+#if SE_TBL
+    const u32 ba = se_tuple(ta) * SE_TBL_R, bb = se_tuple(tb) * SE_TBL_R;
+    ea = ltbl[ba];
+    eb = ltbl[bb];
+    if (!((ea & eb) >> 31)) {
+        u32 ha[4], hb[4];
+        se_hash_lanes(ha, xa, kf, gya);
+        se_hash_lanes(hb, xa, kf, gyb);
+        const u32 ra = (ea >> 31) ? 0u : se_rclass(ha[0], ha[1], ha[2], ha[3]);
+        const u32 rb = (eb >> 31) ? 0u : se_rclass(hb[0], hb[1], hb[2], hb[3]);
+        ea = ltbl[ba + ra];
+        eb = ltbl[bb + rb];
+    }
+    ea &= 0x7fffffffu;
+    eb &= 0x7fffffffu;
+#else
+    (void)ltbl; (void)ta; (void)tb; (void)xa; (void)kf; (void)gya; (void)gyb;
+    ea = eb = 0u;
+#endif
+}
+
+// Chain path: the generated rule chain for one packed block (all-EMPTY blocks
+// skip it).
+__device__ __forceinline__ u32 se_blk_eval1(u32 t, const u32* xa, const u32* kf, int gy) {
+    if (t == 0u) return 0u;
+    u32 h[4];
+    se_hash_lanes(h, xa, kf, gy);
+    u32 c0 = t & 0xffu, c1 = (t >> 8) & 0xffu, c2 = (t >> 16) & 0xffu, c3 = t >> 24;
+    se_block_h(c0, c1, c2, c3, h[0], h[1], h[2], h[3]);
+    return se_pack4(c0, c1, c2, c3);
+}
+
 // One Margolus sub-step over the block region (fx + 2*bx, fy + 2*by), bx < nbx,
 // by < nby, of an LDS tile with row length LW, without modifications.  Lanes of
 // a wavefront take consecutive blocks of one block row (64 per pass), the
@@ -347,14 +429,13 @@ __device__ __forceinline__ u32 se_tuple(u32 t) {
 // column terms (LDS column, the hash's x term) and the row terms are uniform.
 // With x offset 0 a block's cell pairs are 2-byte aligned and move as 16-bit
 // LDS accesses; with x offset 1 they move as bytes (misaligned 16-bit LDS
-// accesses are much slower).  In a tile that reaches past the world (`edge`), cells
-// outside the world stay WALL: the result bytes of those cells are replaced by
-// WALL before the store.
-template <int LW, int NT>
-__device__ __forceinline__ void se_lds_substep(unsigned char* lds, const u32* __restrict__ ltbl, int fx, int nbx,
-                                               int fy, int nby, int gx0, int gy0, int W, int H, u32 f, bool edge) {
+// accesses are much slower).  In a tile that reaches past the world (EDGE),
+// cells outside the world stay WALL: the result bytes of those cells are
+// replaced by WALL before the store.
+template <int LW, int NT, bool ODD, bool EDGE>
+__device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* __restrict__ ltbl, int fx, int nbx,
+                                                 int fy, int nby, int gx0, int gy0, int W, int H, u32 f) {
     constexpr int NW = NT / 64;
-    const bool odd = (fx & 1) != 0;  // block origins share the parity of the x offset
     const int lane = (int)(threadIdx.x & 63);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     u32 kf[4];
@@ -369,59 +450,48 @@ __device__ __forceinline__ void se_lds_substep(unsigned char* lds, const u32* __
 #pragma unroll
         for (int l = 0; l < 4; l++) xa[l] = (u32)gx * se_hash_ka(l);
         u32 colkeep = 0xffffffffu;
-        if (edge)
+        if (EDGE)
             colkeep = ((gx >= 0 && gx < W) ? 0x00ff00ffu : 0u) | ((gx + 1 >= 0 && gx + 1 < W) ? 0xff00ff00u : 0u);
-        for (int by = wv; by < nby; by += NW) {
-            const int ly = fy + 2 * by;
-            const int gy = gy0 + ly;
-            unsigned char* p0 = lds + ly * LW + lx;
-            u32 t;
-            if (odd)
-                t = se_pack4(p0[0], p0[1], p0[LW], p0[LW + 1]);
-            else
-                t = (u32)*reinterpret_cast<const unsigned short*>(p0) |
-                    ((u32)*reinterpret_cast<const unsigned short*>(p0 + LW) << 16);
-            u32 h[4];
-#pragma unroll
-            for (int l = 0; l < 4; l++) h[l] = 0u;
-            u32 e;
+        // table path: two block rows (by, by + NW) per iteration, their reads,
+        // lookups and writes interleaved so the two LDS chains overlap; the
+        // branchy rule chain runs one block row per iteration (measured faster)
+        constexpr int RPI = SE_TBL ? 2 : 1;
+        for (int by = wv; by < nby; by += RPI * NW) {
+            const bool two = RPI == 2 && by + NW < nby;  // uniform
+            const int gya = gy0 + fy + 2 * by, gyb = gya + 2 * NW;
+            unsigned char* pa = lds + (fy + 2 * by) * LW + lx;
+            unsigned char* pb = two ? pa + 2 * NW * LW : pa;
+            const u32 ta = se_blk_read<LW, ODD>(pa), tb = RPI == 2 ? se_blk_read<LW, ODD>(pb) : 0u;
+            u32 ea, eb;
 #if SE_TBL
-            const u32 base = se_tuple(t) * SE_TBL_R;
-            e = ltbl[base];
-            if (!(e >> 31)) {  // the result depends on the block's hash classes
-#pragma unroll
-                for (int l = 0; l < 4; l++)
-                    if (se_hash_used(l)) h[l] = se_lowbias(xa[l] + (u32)gy * se_hash_kb(l) + kf[l]);
-                e = ltbl[base + se_rclass(h[0], h[1], h[2], h[3])];
-            }
-            e &= 0x7fffffffu;
+            se_blk_eval2(ltbl, ta, tb, xa, kf, gya, gyb, ea, eb);
 #else
-            e = t;
-            if (t != 0u) {
-#pragma unroll
-                for (int l = 0; l < 4; l++)
-                    if (se_hash_used(l)) h[l] = se_lowbias(xa[l] + (u32)gy * se_hash_kb(l) + kf[l]);
-                u32 c0 = t & 0xffu, c1 = (t >> 8) & 0xffu, c2 = (t >> 16) & 0xffu, c3 = t >> 24;
-                se_block_h(c0, c1, c2, c3, h[0], h[1], h[2], h[3]);
-                e = se_pack4(c0, c1, c2, c3);
-            }
+            (void)tb;
+            eb = 0u;
+            ea = se_blk_eval1(ta, xa, kf, gya);
 #endif
-            if (edge) {
-                const u32 rowkeep =
-                    ((gy >= 0 && gy < H) ? 0x0000ffffu : 0u) | ((gy + 1 >= 0 && gy + 1 < H) ? 0xffff0000u : 0u);
-                const u32 keep = colkeep & rowkeep;
-                e = (e & keep) | (0x02020202u & ~keep);
+            if (EDGE) {
+                ea = se_keep_wall(ea, colkeep, gya, H);
+                eb = se_keep_wall(eb, colkeep, gyb, H);
             }
-            if (odd) {
-                p0[0] = (unsigned char)e;
-                p0[1] = (unsigned char)(e >> 8);
-                p0[LW] = (unsigned char)(e >> 16);
-                p0[LW + 1] = (unsigned char)(e >> 24);
-            } else {
-                *reinterpret_cast<unsigned short*>(p0) = (unsigned short)e;
-                *reinterpret_cast<unsigned short*>(p0 + LW) = (unsigned short)(e >> 16);
-            }
+            se_blk_write<LW, ODD>(pa, ea);
+            if (two) se_blk_write<LW, ODD>(pb, eb);
         }
+    }
+}
+
+// One Margolus sub-step without modifications; odd x offsets and edge tiles
+// get their own instantiations.
+template <int LW, int NT>
+__device__ __forceinline__ void se_lds_substep(unsigned char* lds, const u32* __restrict__ ltbl, int fx, int nbx,
+                                               int fy, int nby, int gx0, int gy0, int W, int H, u32 f, bool edge) {
+    const bool odd = (fx & 1) != 0;  // block origins share the parity of the x offset
+    if (odd) {
+        if (edge) se_lds_substep_t<LW, NT, true, true>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
+        else se_lds_substep_t<LW, NT, true, false>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
+    } else {
+        if (edge) se_lds_substep_t<LW, NT, false, true>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
+        else se_lds_substep_t<LW, NT, false, false>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
     }
 }
 
@@ -431,7 +501,11 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                                             const se_mod* __restrict__ mods, int nmods,
                                             const u32* __restrict__ ltbl, unsigned char* lds) {
     constexpr int LW = TW + 2 * HB, LH = TH + 2 * HB, LQ = LW / 4;
-    const int tid = threadIdx.x;
+    // The thread index passes through an empty asm so that per-thread offsets of
+    // the tile load/store are recomputed per tile: hoisted out of a persistent
+    // workgroup's tile loop they would stay live in ~80 VGPRs and cut occupancy.
+    int tid = (int)threadIdx.x;
+    __asm__ volatile("" : "+v"(tid));
     const int tx = tile % tiles_x, ty = tile / tiles_x;
     const int gx0 = tx * TW - HB, gy0 = ty * TH - HB;
     const bool interior = gx0 >= 0 && gy0 >= 0 && gx0 + LW <= W && gy0 + LH <= H;
@@ -444,17 +518,14 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
             uint4 v[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int i = i0 + u * NT + tid;
-                if (i < TOTAL) {
-                    const int ly = i / LQ, lq = i - ly * LQ;
-                    v[u] = *reinterpret_cast<const uint4*>(in + (size_t)(gy0 + ly) * pitch + gx0 + 4 * lq);
-                }
+                const int i = min(i0 + u * NT + tid, TOTAL - 1);  // clamped: unconditional loads
+                const int ly = i / LQ, lq = i - ly * LQ;
+                v[u] = *reinterpret_cast<const uint4*>(in + (size_t)(gy0 + ly) * pitch + gx0 + 4 * lq);
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int i = i0 + u * NT + tid;
-                if (i < TOTAL)
-                    lds32[i] = se_pack4(se_decode(v[u].x), se_decode(v[u].y), se_decode(v[u].z), se_decode(v[u].w));
+                if (i < TOTAL) lds32[i] = se_pack_quad(v[u]);
             }
         }
     } else {
@@ -608,7 +679,7 @@ __device__ __forceinline__ const u32* se_stage_table(const u32* __restrict__ tbl
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int i = i0 + u * nt + (int)threadIdx.x;
-            if (i < Q) v[u] = src[i];
+            v[u] = src[min(i, Q - 1)];  // unconditional (clamped) loads keep v[] in registers
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -659,7 +730,9 @@ __device__ __forceinline__ int se_xcd_tile(int b, int n) {
 }
 
 // dynamic LDS: tile bytes rounded to 16, then the table (SE_TBL)
-//  se_step_lds_*  : one world, one workgroup per tile.
+//  se_step_lds_*  : one world, one workgroup per tile (workgroups that start as
+//                   others retire keep tile loads and tile compute overlapped;
+//                   measured faster than persistent workgroups, which move in step).
 //  se_batch_lds_* : persistent over many worlds: workgroups stride over the
 //                   tiles of worlds [0, n_active) (sorted by steps, longest first)
 //                   and advance each by min(k, steps - round*k) sub-steps; a world
@@ -671,8 +744,7 @@ __device__ __forceinline__ int se_xcd_tile(int b, int n) {
         extern __shared__ __align__(16) unsigned char se_lds[];                                               \
         const u32* lt = se_stage_table(tbl, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT);                      \
         se_lds_body<TW, TH, HB, NT>(in, out, W, H, pitch, frame0, nsub, tiles_x,                               \
-                                    se_xcd_tile((int)blockIdx.x, (int)gridDim.x), mods, nmods,                  \
-                                    lt, se_lds);                                                              \
+                                    se_xcd_tile((int)blockIdx.x, (int)gridDim.x), mods, nmods, lt, se_lds);     \
     }                                                                                                         \
     extern "C" __global__ __launch_bounds__(NT) void se_batch_lds_##TW##x##TH##_h##HB##_t##NT(               \
         const se_bworld* __restrict__ worlds, const int* __restrict__ prefix, int n_active, int round, int k, \

@@ -48,8 +48,8 @@ char* dupstr(const std::string& s) {
 
 const std::vector<se::LdsVariant>& lds_variants() {
     static const std::vector<se::LdsVariant> v = {
-        {240, 112, 8, 512}, {240, 240, 8, 512}, {240, 176, 8, 512}, {112, 112, 8, 256}, {224, 224, 16, 512},
-        {192, 192, 8, 512}};
+        {112, 112, 8, 256}, {240, 112, 8, 512}, {224, 112, 8, 512}, {96, 112, 8, 256}, {224, 224, 16, 512},
+        {96, 96, 8, 256}};
     return v;
 }
 
@@ -166,9 +166,13 @@ int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
         std::string bname = se::lds_kernel_name(v);
         bname.replace(0, std::strlen("se_step_lds_"), "se_batch_lds_");
         HIPCHECK(hipModuleGetFunction(&fb, m->mod, bname.c_str()));
+        int per_cu = 0, cus = 0;
+        HIPCHECK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, v.nt, bytes));
+        HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
         m->lds.push_back({v, f});
         m->batch.push_back(fb);
         m->lds_bytes.push_back(bytes);
+        m->lds_grid.push_back(std::max(8, (std::max(per_cu, 1) * cus) & ~7));
     }
     if (m->lds.empty()) return fail(SE_ERR_JIT, "no LDS kernel variant fits the device's LDS");
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -564,6 +568,12 @@ int se_variant_info(const se_world* w, int32_t kind, int32_t index, int32_t* a, 
         return SE_OK;
     }
     if (index < 0 || index >= (int)w->mod->lds.size()) return fail(SE_ERR_ARG, "lds variant out of range");
+    if (kind == 3) {  // launch shape: persistent workgroups, dynamic LDS bytes
+        if (a) *a = w->mod->lds_grid[index];
+        if (b) *b = (int32_t)w->mod->lds_bytes[index];
+        if (c) *c = 0;
+        return SE_OK;
+    }
     const se::LdsVariant& v = w->mod->lds[index].first;
     if (a) *a = v.tw;
     if (b) *b = v.th;

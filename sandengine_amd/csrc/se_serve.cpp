@@ -365,16 +365,12 @@ int enqueue_round(se_serve* S, Slot& sl, std::vector<size_t>& active, bool* any)
         HIPCHECK(hipMemcpyAsync(sl.d_mods, sl.p_mods, sl.h_mods.size() * sizeof(se_modification),
                                 hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(sl.d_pre, sl.p_pre, sl.h_pre.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    int cus = 256, lds_cu = 160 * 1024;
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, S->ctx->device);
-    hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, S->ctx->device);
     for (size_t gi = 0; gi < ranges.size(); gi++) {
         se_module* mod = ranges[gi].first;
         int vi = lds_index(mod);
         const se::LdsVariant& v = mod->lds[vi].first;
-        int per_cu = std::max(1, std::min(lds_cu / (int)mod->lds_bytes[vi], 2048 / v.nt));
         size_t b0 = ranges[gi].second.first, n = ranges[gi].second.second - b0;
-        rc = launch_ex(mod->batch[vi], (unsigned)(cus * per_cu), (unsigned)v.nt, mod->lds_bytes[vi], st,
+        rc = launch_ex(mod->batch[vi], (unsigned)mod->lds_grid[vi], (unsigned)v.nt, mod->lds_bytes[vi], st,
                        (const void*)((BWorld*)sl.d_recs + b0), (const int*)((int*)sl.d_pre + pre_off[gi]), (int)n, 0,
                        (int)(1 << 30), (const uint32_t*)mod->table);
         if (rc) return rc;

@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 FILLS = {"four_materials": ([0, 3, 4, 5], [50, 25, 15, 10]),
          "twelve_materials": ([0] + list(range(3, 15)), [40] + [5] * 12)}
 SHAPES = [(2, 2), (1, 7), (64, 64), (100, 36), (192, 192), (194, 190), (200, 200), (256, 256), (300, 140),
-          (513, 257), (1024, 64), (70, 700), (386, 386), (130, 1030)]
+          (513, 257), (1024, 64), (70, 700), (386, 386), (130, 1030), (112, 112), (114, 110), (224, 113)]
 _ctx = None
 _rs = {}
 

@@ -79,7 +79,7 @@ def test_golden(case, mode, sub):
 
 
 RANDOM_SHAPES = [(1, 1), (2, 1), (3, 7), (255, 3), (256, 5), (257, 9), (300, 301), (513, 64), (700, 33),
-                 (1030, 17), (64, 1000)]
+                 (1030, 17), (64, 1000), (112, 112), (226, 114), (97, 250)]
 
 
 @pytest.mark.parametrize("rsname", ["four", "twelve", "features"])
