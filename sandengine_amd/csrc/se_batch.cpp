@@ -193,9 +193,7 @@ int run_batched(se_batch* b, const std::vector<std::pair<se_world*, uint32_t>>& 
                 vi = t->second.first;
                 if (kk == 0) kk = t->second.second;
             } else {
-                vi = 0;
-                for (size_t i = 0; i < mod->lds.size(); i++)
-                    if (mod->lds[i].first.tw == 192 && mod->lds[i].first.hb == 8) vi = (int)i;
+                vi = se_rt::default_lds_index(mod);
             }
         }
         if (vi < 0 || vi >= (int)mod->lds.size()) return fail(SE_ERR_ARG, "LDS variant out of range");

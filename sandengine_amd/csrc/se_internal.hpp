@@ -116,4 +116,7 @@ struct se_world {
 namespace se_rt {
 // One Simulation::run of `w` with the plain kernel (frame += 1, mods, step).
 int plain_step(se_world* w);
+// Index of the default LDS tile variant of a module (batches and serving use it
+// unless tuned).
+int default_lds_index(const se_module* m);
 }  // namespace se_rt

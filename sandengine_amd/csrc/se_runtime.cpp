@@ -256,6 +256,19 @@ using se_rt::plain_step;
 // >= sub.
 constexpr se::LdsVariant kDefaultLds = {112, 112, 8, 256};
 
+}  // namespace
+
+int se_rt::default_lds_index(const se_module* m) {
+    for (size_t i = 0; i < m->lds.size(); i++) {
+        const se::LdsVariant& v = m->lds[i].first;
+        if (v.tw == kDefaultLds.tw && v.th == kDefaultLds.th && v.hb == kDefaultLds.hb && v.nt == kDefaultLds.nt)
+            return (int)i;
+    }
+    return 0;
+}
+
+namespace {
+
 const std::pair<se::LdsVariant, hipFunction_t>* pick_lds(se_world* w, uint32_t sub) {
     auto& L = w->mod->lds;
     if (w->lds_variant >= 0) {
@@ -551,34 +564,27 @@ int se_world_set_variant(se_world* w, int32_t plain_variant, int32_t lds_variant
 
 int se_variant_info(const se_world* w, int32_t kind, int32_t index, int32_t* a, int32_t* b, int32_t* c) {
     if (!w) return fail(SE_ERR_ARG, "null world");
+    auto put = [&](int32_t x, int32_t y, int32_t z) {
+        if (a) *a = x;
+        if (b) *b = y;
+        if (c) *c = z;
+        return SE_OK;
+    };
     if (kind == 0) {
         if (index < 0 || index >= 6) return fail(SE_ERR_ARG, "plain variant out of range");
         static const int kRows[3] = {1, 2, 4};
-        if (a) *a = kRows[index % 3];
-        if (b) *b = index / 3;
-        if (c) *c = 0;
-        return SE_OK;
+        return put(kRows[index % 3], index / 3, 0);
     }
     if (kind == 2) {  // the LDS variant se_world_step_blocked(w, n, index) launches
         auto* p = pick_lds(const_cast<se_world*>(w), (uint32_t)std::max(index, 0));
         if (!p) return fail(SE_ERR_ARG, "no LDS variant supports " + std::to_string(index) + " sub-steps");
-        if (a) *a = (int32_t)(p - &w->mod->lds[0]);
-        if (b) *b = 0;
-        if (c) *c = 0;
-        return SE_OK;
+        return put((int32_t)(p - &w->mod->lds[0]), 0, 0);
     }
     if (index < 0 || index >= (int)w->mod->lds.size()) return fail(SE_ERR_ARG, "lds variant out of range");
-    if (kind == 3) {  // launch shape: persistent workgroups, dynamic LDS bytes
-        if (a) *a = w->mod->lds_grid[index];
-        if (b) *b = (int32_t)w->mod->lds_bytes[index];
-        if (c) *c = 0;
-        return SE_OK;
-    }
     const se::LdsVariant& v = w->mod->lds[index].first;
-    if (a) *a = v.tw;
-    if (b) *b = v.th;
-    if (c) *c = v.hb * 1000 + v.nt;
-    return SE_OK;
+    if (kind == 1) return put(v.tw, v.th, v.hb * 1000 + v.nt);
+    if (kind == 3) return put(w->mod->lds_grid[index], (int32_t)w->mod->lds_bytes[index], 0);
+    return fail(SE_ERR_ARG, "unknown variant kind");
 }
 
 int se_world_get_frame(const se_world* w, int32_t* frame) {

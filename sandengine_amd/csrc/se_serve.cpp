@@ -266,11 +266,7 @@ uint32_t quantum(const se_serve* S, const Request& r, const se::LdsVariant& v) {
     return q;
 }
 
-int lds_index(const se_module* m) {
-    for (size_t i = 0; i < m->lds.size(); i++)
-        if (m->lds[i].first.tw == 192 && m->lds[i].first.th == 192 && m->lds[i].first.hb == 8) return (int)i;
-    return 0;
-}
+int lds_index(const se_module* m) { return se_rt::default_lds_index(m); }
 
 // Builds and enqueues one round over every admitted world with steps left.
 int enqueue_round(se_serve* S, Slot& sl, std::vector<size_t>& active, bool* any) {
