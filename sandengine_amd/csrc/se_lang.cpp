@@ -168,7 +168,7 @@ void parse_conditionals(const YNode& parent, bool parent_is_else, const std::str
 std::vector<std::string> preparse_keys(const YNode& m, const std::string& err) {
     std::vector<std::string> out;
     for (size_t i = 0; i < m.map.size(); i++) {
-        if (!m.key_is_str[i]) throw invalid(m.map[i].first, err, kHintString);
+        if (!m.key_is_str[i]) throw invalid(m.key_dbg[i], err, kHintString);
         out.push_back(m.map[i].first);
     }
     return out;
@@ -185,7 +185,7 @@ std::vector<SandRule> parse_rules(const YNode& rules, const std::vector<std::str
                                   const std::vector<std::string>& mat_names) {
     std::vector<SandRule> out;
     for (size_t i = 0; i < rules.map.size(); i++) {
-        if (!rules.key_is_str[i]) throw invalid(rules.map[i].first, "rules", kHintString);
+        if (!rules.key_is_str[i]) throw invalid(rules.key_dbg[i], "rules", kHintString);
         const std::string& name = rules.map[i].first;
         const YNode& body = rules.map[i].second;
         SandRule r;
@@ -253,7 +253,7 @@ std::vector<SandType> parse_types(const YNode& types, std::vector<SandRule>& rul
     ts[2].id = 2; ts[2].name = "WALL";
     size_t idx = ts.size();
     for (size_t i = 0; i < types.map.size(); i++) {
-        if (!types.key_is_str[i]) throw invalid(types.map[i].first, "types", kHintString);
+        if (!types.key_is_str[i]) throw invalid(types.key_dbg[i], "types", kHintString);
         const std::string& name = types.map[i].first;
         const YNode& body = types.map[i].second;
         std::string parent;
@@ -329,7 +329,7 @@ std::vector<SandMaterial> parse_materials(const YNode& mats, std::vector<SandRul
     size_t idx = ms.size();
     static const float kColorDef[4] = {1, 0, 1, 1}, kEmisDef[4] = {0, 0, 0, 0};
     for (size_t i = 0; i < mats.map.size(); i++) {
-        if (!mats.key_is_str[i]) throw invalid(mats.map[i].first, "materials", kHintString);
+        if (!mats.key_is_str[i]) throw invalid(mats.key_dbg[i], "materials", kHintString);
         const std::string& name = mats.map[i].first;
         const YNode& body = mats.map[i].second;
         SandMaterial m;

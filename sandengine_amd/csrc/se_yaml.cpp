@@ -201,6 +201,7 @@ private:
             }
             n.map.emplace_back(key, std::move(v));
             n.key_is_str.push_back(kstr);
+            n.key_dbg.push_back(k.debug());
         }
         if (i_ < lines_.size() && lines_[i_].indent > ind) fail(lines_[i_].no, "bad indentation");
         return n;
@@ -319,7 +320,8 @@ private:
                 for (auto& kv : n.map)
                     if (kv.first == key) fail(no, "duplicate entry with key \"" + key + "\"");
                 n.map.emplace_back(key, std::move(v));
-                n.key_is_str.push_back(k.kind == YNode::Scalar && (k.quoted || (!k.is_int() && !k.is_float())));
+                n.key_is_str.push_back(k.as_str(nullptr));
+                n.key_dbg.push_back(k.debug());
                 skipws(s, p);
                 if (p < s.size() && s[p] == ',') { p++; continue; }
                 if (p < s.size() && s[p] == '}') { p++; break; }

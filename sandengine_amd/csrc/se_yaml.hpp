@@ -27,6 +27,7 @@ struct YNode {
     std::vector<YNode> seq;
     std::vector<std::pair<std::string, YNode>> map;  // keys are scalar texts
     std::vector<bool> key_is_str;                    // key typed as a string?
+    std::vector<std::string> key_dbg;                // key rendered like a serde_yaml Value ({:?})
     int line = 0;
 
     bool is_null() const;

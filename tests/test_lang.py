@@ -80,6 +80,14 @@ ERROR_CASES = [
     ("rules: {}\ntypes:\n  t:\nmaterials:\n  m:\n    type: t\n    color: [1, 2, 3]\n", "MissingField"),
     ("rules: {}\ntypes:\n  t:\nmaterials:\n  m:\n    type: t\n    color: [1, 2, 300]\n    density: 1\n",
      "InvalidType"),
+    # the situations tests/test_sandengine-lang.rs names, as the current parser.rs:91-152
+    # reports them: the three sections are looked up before any is parsed
+    ("rules:\n  2.5:\n    if: isType_t(DOWN)\n    do: SWAP SELF DOWN\ntypes:\n  t:\nmaterials: {}\n",
+     "InvalidType"),
+    ("rules:\n  2.5:\n    if: isType_t(DOWN)\n    do: SWAP SELF DOWN\n", "MissingField"),
+    ("rules:\n  r:\n    if: isType_t(DOWN)\n    do: SWAP SELF DOWN\nmaterials: {}\n", "MissingField"),
+    ("rules:\n  r:\n    if: isType_t(DOWN)\n    do: SWAP SELF DOWN\ntypes:\n  t:\n", "MissingField"),
+    ("rules: {}\ntypes:\n  t:\nmaterials:\n  m:\n    type: t\n    density: 1\n", "MissingField"),
 ]
 
 
