@@ -124,6 +124,18 @@ int se_world_step_blocked(se_world* w, uint32_t nsteps, uint32_t sub);
  * serving loop's persistent grid), b=its dynamic LDS bytes). */
 int se_world_set_variant(se_world* w, int32_t plain_variant, int32_t lds_variant);
 int se_variant_info(const se_world* w, int32_t kind, int32_t index, int32_t* a, int32_t* b, int32_t* c);
+/* World save/load (the reference lists "World loading/ unloading -- using
+ * chunks, save all relevant textures" as a to-do, README.md:85-86): the grid,
+ * the frame and the queued modifications, in a binary file written and read in
+ * chunks of rows.  RAND is a function of (block origin, frame) only, so a loaded
+ * world continues exactly as the saved one would have.  Layout: 40-byte header
+ * {char magic[8] = "SEWORLD1"; u32 version = 1, W, H; i32 frame; u32
+ * n_modifications, reserved; u64 FNV-1a 64 of the rule set's IR JSON}, then
+ * W*H u32 cell ids (row-major, little-endian), then n_modifications records of
+ * se_modification.  se_world_load creates the world; a file saved with another
+ * rule set, a short file or a bad header is refused (SE_ERR_ARG). */
+int se_world_save(se_world* w, const char* path);
+int se_world_load(se_ctx* ctx, const se_ruleset* rs, const char* path, se_world** out);
 /* Params.frame (simulation.rs:78, 201). Negative frames are rejected. */
 int se_world_get_frame(const se_world* w, int32_t* frame);
 int se_world_set_frame(se_world* w, int32_t frame);

@@ -21,8 +21,10 @@ raises ``SandengineError``.
 from __future__ import annotations
 
 import ctypes
+import math
 import json
 import os
+import struct
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence, Tuple
 
@@ -97,6 +99,8 @@ def lib():
         L.se_world_step.argtypes = [c_p, c_u32]
         L.se_world_step_blocked.argtypes = [c_p, c_u32, c_u32]
         L.se_world_get_frame.argtypes = [c_p, ctypes.POINTER(c_i32)]
+        L.se_world_save.argtypes = [c_p, ctypes.c_char_p]
+        L.se_world_load.argtypes = [c_p, c_p, ctypes.c_char_p, PP]
         L.se_world_set_frame.argtypes = [c_p, c_i32]
         L.se_world_sync.argtypes = [c_p]
         L.se_world_device_ptr.argtypes = [c_p, PP, ctypes.POINTER(c_u32)]
@@ -356,6 +360,21 @@ class World:
         _check(lib().se_variant_info(self._p, kind, index, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
 
+    def save(self, path: str) -> None:
+        """Writes the grid, frame and queued modifications (se_world_save)."""
+        _check(lib().se_world_save(self._p, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, ctx: "Context", ruleset: "RuleSet", path: str) -> "World":
+        """A world saved by save(); the rule set must be the one it was saved with."""
+        p = ctypes.c_void_p()
+        _check(lib().se_world_load(ctx._p, ruleset._p, os.fsencode(path), ctypes.byref(p)))
+        with open(path, "rb") as f:
+            W, H = struct.unpack("<8sIII", f.read(20))[2:4]
+        w = cls.__new__(cls)
+        w._p, w.ctx, w.rules, w.W, w.H = p, ctx, ruleset, W, H
+        return w
+
     @property
     def frame(self) -> int:
         v = ctypes.c_int32()
@@ -534,8 +553,14 @@ class SimModification:
 
 @dataclass
 class Params:
-    """The parts of simulation.rs:69-91 Params the step reads."""
+    """simulation.rs:69-91 Params: the frame the step reads, and the brush state
+    the input loop keeps (sandengine-core/src/lib.rs:57-120)."""
     frame: int = 0
+    mousePos: Tuple[float, float] = (0.0, 0.0)
+    mousePressed: bool = False
+    brushSize: int = 5
+    brushMaterial: Optional[SandMaterial] = None  # None: SandMaterial::default(), id 0
+    time: float = 0.0
 
 
 class Simulation:
@@ -568,3 +593,63 @@ class Simulation:
 
     def upload(self, grid: np.ndarray):
         self.world.upload(grid)
+
+
+def _f32_to_i32(v) -> int:
+    """Rust `f32 as i32`: truncation toward zero, saturating, NaN -> 0."""
+    v = float(v)
+    if v != v:
+        return 0
+    if v >= 2147483647.0:
+        return 2147483647
+    if v <= -2147483648.0:
+        return -2147483648
+    return int(v)
+
+
+class InputLoop:
+    """Headless form of sandengine_core::run's event loop
+    (sandengine-core/src/lib.rs:19-120): scripted window events drive the same
+    brush state, and frame() is one MainEventsCleared -- a circle brush at the
+    cursor while the left button is held, then Simulation.run().  Rendering,
+    the UI and frame timing are not part of it.
+
+    key(k)                 number keys 0-9 pick selectable_materials[k]; any
+                           other key picks index 0 (lib.rs:87-101)
+    cursor_moved(x, y, d)  mousePos = position / framebuffer size, in f32 (lib.rs:103-106)
+    mouse_left(pressed)    left button state (lib.rs:107-113)
+    wheel(lines)           brushSize = max(1, brushSize + signum(lines)) (lib.rs:114-118)
+    """
+
+    def __init__(self, sim: "Simulation", parsing_result: ParsingResult):
+        self.sim = sim
+        self.selectable = [m for m in parsing_result.materials if m.selectable]  # lib.rs:21-27
+
+    def key(self, key) -> None:
+        digits = {str(d): d for d in range(10)}
+        idx = key if isinstance(key, int) and 0 <= key <= 9 else digits.get(str(key), 0)
+        if idx < len(self.selectable):
+            self.sim.params.brushMaterial = self.selectable[idx]
+
+    def cursor_moved(self, x: float, y: float, framebuffer_dims: Tuple[int, int]) -> None:
+        f = np.float32
+        self.sim.params.mousePos = (float(f(x) / f(framebuffer_dims[0])), float(f(y) / f(framebuffer_dims[1])))
+
+    def mouse_left(self, pressed: bool) -> None:
+        self.sim.params.mousePressed = bool(pressed)
+
+    def wheel(self, lines_y: float) -> None:
+        y = float(lines_y)
+        sign = 0 if y != y else (1 if math.copysign(1.0, y) > 0 else -1)  # f32::signum: +0.0 -> 1
+        self.sim.params.brushSize = max(1, int(self.sim.params.brushSize) + sign)
+
+    def frame(self) -> None:
+        p = self.sim.params
+        if p.mousePressed:
+            f = np.float32
+            self.sim.modifications.append(SimModification(
+                position=(_f32_to_i32(f(p.mousePos[0]) * f(self.sim.size[0])),
+                          _f32_to_i32(f(p.mousePos[1]) * f(self.sim.size[1]))),
+                mod_shape=MODSHAPE_CIRCLE, mod_size=int(p.brushSize),
+                mod_matID=p.brushMaterial.id if p.brushMaterial is not None else 0))
+        self.sim.run()

@@ -587,6 +587,106 @@ int se_variant_info(const se_world* w, int32_t kind, int32_t index, int32_t* a, 
     return fail(SE_ERR_ARG, "unknown variant kind");
 }
 
+// ------------------------------------------------------------------ save / load
+// File: a 40-byte header (magic "SEWORLD1", version 1, W, H, frame, number of
+// queued modifications, a reserved word, FNV-1a 64 of the rule set's IR), the
+// grid as W*H little-endian u32 (row-major), then the queued 32-byte
+// modifications.  The grid moves in chunks of whole rows through a host buffer
+// of at most 16 MiB.
+namespace {
+struct WorldFileHeader {
+    char magic[8];
+    uint32_t version, W, H;
+    int32_t frame;
+    uint32_t npending, reserved;
+    uint64_t ruleset_hash;
+};
+static_assert(sizeof(WorldFileHeader) == 40, "world file header layout");
+const char kWorldMagic[8] = {'S', 'E', 'W', 'O', 'R', 'L', 'D', '1'};
+
+uint64_t ruleset_hash(const se_ruleset* rs) {
+    const std::string j = se::ir_json(rs->pr);
+    return se_fnv1a64(j.data(), j.size());
+}
+
+size_t chunk_rows(uint32_t W) { return std::max<size_t>(1, (16u << 20) / ((size_t)W * 4)); }
+
+struct File {
+    FILE* f = nullptr;
+    ~File() {
+        if (f) std::fclose(f);
+    }
+};
+}  // namespace
+
+int se_world_save(se_world* w, const char* path) {
+    if (!w || !path) return fail(SE_ERR_ARG, "null argument");
+    File F;
+    F.f = std::fopen(path, "wb");
+    if (!F.f) return fail(SE_ERR_ARG, std::string("cannot open '") + path + "' for writing");
+    WorldFileHeader h{};
+    std::memcpy(h.magic, kWorldMagic, 8);
+    h.version = 1;
+    h.W = w->W;
+    h.H = w->H;
+    h.frame = w->frame;
+    h.npending = (uint32_t)w->pending.size();
+    h.ruleset_hash = ruleset_hash(w->rs);
+    if (std::fwrite(&h, sizeof h, 1, F.f) != 1) return fail(SE_ERR_ARG, "write failed");
+    DeviceGuard dg(w->ctx->device);
+    const size_t rows = chunk_rows(w->W);
+    std::vector<uint32_t> buf(rows * w->W);
+    for (size_t y = 0; y < w->H; y += rows) {
+        const size_t n = std::min<size_t>(rows, w->H - y);
+        HIPCHECK(hipMemcpy2DAsync(buf.data(), (size_t)w->W * 4, w->buf[w->cur] + y * w->pitch, (size_t)w->pitch * 4,
+                                  (size_t)w->W * 4, n, hipMemcpyDeviceToHost, w->stream));
+        HIPCHECK(hipStreamSynchronize(w->stream));
+        if (std::fwrite(buf.data(), 4, n * w->W, F.f) != n * w->W) return fail(SE_ERR_ARG, "write failed");
+    }
+    if (!w->pending.empty() &&
+        std::fwrite(w->pending.data(), sizeof(se_modification), w->pending.size(), F.f) != w->pending.size())
+        return fail(SE_ERR_ARG, "write failed");
+    if (std::fclose(F.f) != 0) {
+        F.f = nullptr;
+        return fail(SE_ERR_ARG, "write failed");
+    }
+    F.f = nullptr;
+    return SE_OK;
+}
+
+int se_world_load(se_ctx* ctx, const se_ruleset* rs, const char* path, se_world** out) {
+    if (!ctx || !rs || !path || !out) return fail(SE_ERR_ARG, "null argument");
+    *out = nullptr;
+    File F;
+    F.f = std::fopen(path, "rb");
+    if (!F.f) return fail(SE_ERR_ARG, std::string("cannot open '") + path + "'");
+    WorldFileHeader h;
+    if (std::fread(&h, sizeof h, 1, F.f) != 1 || std::memcmp(h.magic, kWorldMagic, 8) != 0 || h.version != 1)
+        return fail(SE_ERR_ARG, std::string("'") + path + "' is not a saved world");
+    if (h.ruleset_hash != ruleset_hash(rs)) return fail(SE_ERR_ARG, "the world was saved with a different rule set");
+    if (h.frame < 0 || h.npending > SE_MAX_MODIFICATIONS) return fail(SE_ERR_ARG, "corrupt world header");
+    se_world* w = nullptr;
+    int rc = se_world_create(ctx, rs, h.W, h.H, &w);
+    if (rc) return rc;
+    std::unique_ptr<se_world, void (*)(se_world*)> guard(w, se_world_destroy);
+    DeviceGuard dg(ctx->device);
+    const size_t rows = chunk_rows(w->W);
+    std::vector<uint32_t> buf(rows * w->W);
+    for (size_t y = 0; y < w->H; y += rows) {
+        const size_t n = std::min<size_t>(rows, w->H - y);
+        if (std::fread(buf.data(), 4, n * w->W, F.f) != n * w->W) return fail(SE_ERR_ARG, "truncated world file");
+        HIPCHECK(hipMemcpy2DAsync(w->buf[w->cur] + y * w->pitch, (size_t)w->pitch * 4, buf.data(), (size_t)w->W * 4,
+                                  (size_t)w->W * 4, n, hipMemcpyHostToDevice, w->stream));
+        HIPCHECK(hipStreamSynchronize(w->stream));
+    }
+    w->pending.resize(h.npending);
+    if (h.npending && std::fread(w->pending.data(), sizeof(se_modification), h.npending, F.f) != h.npending)
+        return fail(SE_ERR_ARG, "truncated world file");
+    w->frame = h.frame;
+    *out = guard.release();
+    return SE_OK;
+}
+
 int se_world_get_frame(const se_world* w, int32_t* frame) {
     if (!w || !frame) return fail(SE_ERR_ARG, "null argument");
     *frame = w->frame;

@@ -141,6 +141,101 @@ def test_simulation_api_matches_oracle():
     np.testing.assert_array_equal(sim.grid, want)
 
 
+def test_input_loop_matches_oracle():
+    """Scripted input through the headless event loop: the brush follows the
+    cursor, number keys switch material, the wheel resizes it."""
+    text = S.bundled_ruleset("four_materials")
+    res = S.parse_string(text)
+    blob = O.compile_ruleset(L.parse_string(text))
+    sim = S.Simulation((80, 60), text, ctx=ctx())
+    il = S.InputLoop(sim, res)
+    il.frame()  # frame 1 clears
+    g0 = O.fill(9, 80, 60, [0, 3, 4, 5], [50, 25, 15, 10])
+    sim.upload(g0)
+    want = g0
+    script = {3: ("key", 1), 4: ("move", (200, 100)), 5: ("press", True), 9: ("wheel", 2.0), 12: ("key", 2),
+              13: ("move", (500, 300)), 16: ("press", False), 20: ("key", 0), 21: ("press", True)}
+    for f in range(2, 30):
+        ev = script.get(f)
+        if ev:
+            kind, arg = ev
+            if kind == "key":
+                il.key(arg)
+            elif kind == "move":
+                il.cursor_moved(arg[0], arg[1], (640, 480))
+            elif kind == "press":
+                il.mouse_left(arg)
+            else:
+                il.wheel(arg)
+        queued_before = len(sim.modifications)
+        il.frame()
+        p = sim.params
+        mods = None
+        if p.mousePressed:
+            mx = int(np.float32(np.float32(p.mousePos[0]) * np.float32(80)))
+            my = int(np.float32(np.float32(p.mousePos[1]) * np.float32(60)))
+            mods = [[mx, my, S.MODSHAPE_CIRCLE, p.brushSize, p.brushMaterial.id if p.brushMaterial else 0]]
+        assert queued_before == 0
+        want = O.step(blob, want, f, mods)
+    assert sim.params.frame == 29
+    np.testing.assert_array_equal(sim.grid, want)
+
+
+def test_save_load_continues_exactly(tmp_path):
+    """A world saved mid-run (with modifications queued) and loaded into a new
+    world continues exactly as the original; other rule sets and bad files are
+    refused."""
+    rs4 = ruleset("four")
+    rs12 = ruleset("twelve")
+    W, H = 301, 517
+    g0 = O.fill(21, W, H, [0] + list(range(3, 15)), [40] + [5] * 12)
+    a = S.World(ctx(), rs12, W, H)
+    a.step(1)
+    a.upload(g0)
+    a.frame = 40
+    a.step(13)
+    a.push_modification(150, 200, 0, 30, 9)
+    a.push_modification(10, 10, 1, 5, 0)
+    path = str(tmp_path / "w.sew")
+    a.save(path)
+    b = S.World.load(ctx(), rs12, path)
+    assert (b.W, b.H, b.frame) == (W, H, a.frame)
+    np.testing.assert_array_equal(b.download(), a.download())
+    a.step(20)
+    b.step_blocked(20, 0)
+    np.testing.assert_array_equal(b.download(), a.download())
+    with pytest.raises(S.SandengineError):
+        S.World.load(ctx(), rs4, path)
+    with open(path, "r+b") as f:
+        f.truncate(100)
+    with pytest.raises(S.SandengineError):
+        S.World.load(ctx(), rs12, path)
+    with open(path, "wb") as f:
+        f.write(b"not a world at all, not a world at all")
+    with pytest.raises(S.SandengineError):
+        S.World.load(ctx(), rs12, path)
+    a.close()
+    b.close()
+
+
+def test_save_load_large_world_in_chunks(tmp_path):
+    """4096^2 (64 MiB per grid: several 16 MiB chunks each way)."""
+    rs4 = ruleset("four")
+    a = S.World(ctx(), rs4, 4096, 4096)
+    a.frame = 1
+    _config1(a)
+    a.step(7)
+    path = str(tmp_path / "big.sew")
+    a.save(path)
+    b = S.World.load(ctx(), rs4, path)
+    a.step(9)
+    b.step(9)
+    assert b.frame == a.frame
+    np.testing.assert_array_equal(b.download(), a.download())
+    a.close()
+    b.close()
+
+
 def _config1(w):
     w.fill_seeded(1, [0, 3, 4], [60, 25, 15], stone_rows=(w.H * 3) // 10, stone_id=5)
 
