@@ -124,6 +124,22 @@ int se_world_step_blocked(se_world* w, uint32_t nsteps, uint32_t sub);
  * serving loop's persistent grid), b=its dynamic LDS bytes). */
 int se_world_set_variant(se_world* w, int32_t plain_variant, int32_t lds_variant);
 int se_variant_info(const se_world* w, int32_t kind, int32_t index, int32_t* a, int32_t* b, int32_t* c);
+/* Render side-outputs of every step: the colour and light images that
+ * setCell writes beside the grid (shaders/compute/operations.glsl:99-170;
+ * Simulation::output_color / output_light, simulation.rs:109-114).  Enable before
+ * the first step to match the reference, whose light starts at zero
+ * (simulation.rs:180-181) and then propagates one cell per step.  While
+ * enabled, every step runs the plain kernel and then the render kernel (the
+ * light needs each step's grid), and the world cannot join a batch.  Images are
+ * RGBA f32 per cell, W*H*4 floats row-major on download.  Colour noise uses this
+ * platform's sinf, so it matches a GL implementation only to within its sin
+ * error; the light is plain f32 arithmetic in the shader's order.  The light
+ * returned is the one the last step computed (after Simulation::run's swap the
+ * reference holds it in input_light; the texture named output_light, which its
+ * renderer samples, is then one step older). */
+int se_world_set_render(se_world* w, int enable);
+int se_world_render_download(se_world* w, float* color, float* light);
+int se_world_render_ptrs(se_world* w, void** color, void** light, uint32_t* pitch);
 /* World save/load (the reference lists "World loading/ unloading -- using
  * chunks, save all relevant textures" as a to-do, README.md:85-86): the grid,
  * the frame and the queued modifications, in a binary file written and read in

@@ -45,6 +45,8 @@ def lib(nthreads: int = 0):
         _lib.lp_download.argtypes = [ctypes.POINTER(ctypes.c_uint32)]
         _lib.lp_run.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
         _lib.lp_time_run.restype = ctypes.c_double
+        _lib.lp_download_color.argtypes = [ctypes.POINTER(ctypes.c_float)]
+        _lib.lp_download_light.argtypes = [ctypes.POINTER(ctypes.c_float)]
         if _lib.lp_init(nthreads) != 0:
             raise LlvmpipeError(_lib.lp_error().decode())
     return _lib
@@ -104,6 +106,16 @@ class World:
         if lib().lp_download(g.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))):
             raise LlvmpipeError("download failed")
         return g
+
+    def render(self):
+        """(colour, light) RGBA f32 images of the last step, each (H, W, 4)."""
+        out = []
+        for fn in (lib().lp_download_color, lib().lp_download_light):
+            a = np.zeros((self.H, self.W, 4), np.float32)
+            if fn(a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))):
+                raise LlvmpipeError("render download failed")
+            out.append(a)
+        return tuple(out)
 
     def run(self, nsteps: int, mods=None):
         m = np.zeros((max(1, len(mods or [])), 8), np.int32)

@@ -293,6 +293,17 @@ int lp_download(uint32_t *grid) {
     return glGetError_p() == GL_NO_ERROR ? 0 : -2;
 }
 
+/* RGBA f32 images of the last step: colour (output_color) and light (the
+ * light that step wrote, held in the light input after the swap). */
+static int download_rgba(GLuint tex, float *out) {
+    glFinish_p();
+    glBindTexture_p(GL_TEXTURE_2D, tex);
+    glGetTexImage_p(GL_TEXTURE_2D, 0, GL_RGBA, GL_FLOAT, out);
+    return glGetError_p() == GL_NO_ERROR ? 0 : -2;
+}
+int lp_download_color(float *rgba) { return download_rgba(g_tex_color, rgba); }
+int lp_download_light(float *rgba) { return download_rgba(g_light_in, rgba); }
+
 int lp_get_frame(void) { return g_frame; }
 void lp_set_frame(int f) { g_frame = f; }
 

@@ -100,6 +100,9 @@ def lib():
         L.se_world_step_blocked.argtypes = [c_p, c_u32, c_u32]
         L.se_world_get_frame.argtypes = [c_p, ctypes.POINTER(c_i32)]
         L.se_world_save.argtypes = [c_p, ctypes.c_char_p]
+        L.se_world_set_render.argtypes = [c_p, ctypes.c_int]
+        L.se_world_render_download.argtypes = [c_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+        L.se_world_render_ptrs.argtypes = [c_p, PP, PP, ctypes.POINTER(c_u32)]
         L.se_world_load.argtypes = [c_p, c_p, ctypes.c_char_p, PP]
         L.se_world_set_frame.argtypes = [c_p, c_i32]
         L.se_world_sync.argtypes = [c_p]
@@ -359,6 +362,18 @@ class World:
         a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         _check(lib().se_variant_info(self._p, kind, index, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def set_render(self, enable: bool = True) -> None:
+        """Colour and light side-outputs of every step (se_world_set_render)."""
+        _check(lib().se_world_set_render(self._p, 1 if enable else 0))
+
+    def render_download(self) -> Tuple[np.ndarray, np.ndarray]:
+        """(colour, light) of the last step, each (H, W, 4) float32 RGBA."""
+        col = np.empty((self.H, self.W, 4), np.float32)
+        light = np.empty((self.H, self.W, 4), np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        _check(lib().se_world_render_download(self._p, col.ctypes.data_as(fp), light.ctypes.data_as(fp)))
+        return col, light
 
     def save(self, path: str) -> None:
         """Writes the grid, frame and queued modifications (se_world_save)."""

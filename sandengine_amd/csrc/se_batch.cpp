@@ -293,6 +293,7 @@ int se_batch_add(se_batch* b, se_world* w, uint32_t nsteps) {
     if (!b || !w) return fail(SE_ERR_ARG, "null argument");
     if (w->ctx != b->ctx) return fail(SE_ERR_ARG, "world belongs to another context");
     if ((int64_t)w->frame + nsteps > INT32_MAX) return fail(SE_ERR_ARG, "frame counter would overflow");
+    if (w->color) return fail(SE_ERR_STATE, "a world with render outputs steps alone (one step per launch)");
     for (auto& it : b->items)
         if (it.first == w) return fail(SE_ERR_ARG, "world already in the batch");
     b->items.push_back({w, nsteps});

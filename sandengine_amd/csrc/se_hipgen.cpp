@@ -1,6 +1,7 @@
 #include "se_hipgen.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -41,6 +42,7 @@ struct Ctx {
     std::map<std::string, int> types;  // name -> id
     std::vector<int> mat_type;         // material id -> type id
     std::vector<float> dens;           // material id -> density
+    std::vector<std::array<float, 4>> color, emission;  // material id -> RGBA
     std::vector<std::vector<int>> type_members;  // type id -> {type id} + children type ids
     int nmat = 0;
 };
@@ -585,11 +587,17 @@ Ctx build_ctx(const ParsingResult& r) {
     if (ctx.nmat > 255) gen_error("more than 255 materials are not supported");
     ctx.mat_type.resize(ctx.nmat);
     ctx.dens.resize(ctx.nmat);
+    ctx.color.resize(ctx.nmat);
+    ctx.emission.resize(ctx.nmat);
     for (auto& m : r.materials) {
         auto it = ctx.types.find(m.mattype);
         if (it == ctx.types.end()) gen_error("material '" + m.name + "' has unknown type");
         ctx.mat_type[m.id] = it->second;
         ctx.dens[m.id] = m.density;
+        for (int k = 0; k < 4; k++) {
+            ctx.color[m.id][k] = m.color[k];
+            ctx.emission[m.id][k] = m.emission[k];
+        }
     }
     ctx.type_members.resize(r.types.size());
     for (auto& t : r.types) {
@@ -733,6 +741,28 @@ std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info) {
     o << "__device__ __forceinline__ unsigned int se_type(unsigned int c) {\n    const unsigned int t[" << n << "] = {";
     for (int i = 0; i < n; i++) o << (i ? ", " : "") << ctx.mat_type[i];
     o << "};\n    return t[c];\n}\n";
+    // render side-outputs (operations.glsl:99-170): material colour and emission,
+    // and isLightObstacle (operations.glsl:68-70): no RGB emission and not of type EMPTY
+    for (int k = 0; k < 2; k++) {
+        const auto& tab = k == 0 ? ctx.color : ctx.emission;
+        o << "__device__ __forceinline__ float4 " << (k == 0 ? "se_color" : "se_emission")
+          << "(unsigned int c) {\n    const float4 t[" << n << "] = {";
+        for (int i = 0; i < n; i++)
+            o << (i ? ", " : "") << "{" << flit(tab[i][0]) << ", " << flit(tab[i][1]) << ", " << flit(tab[i][2]) << ", "
+              << flit(tab[i][3]) << "}";
+        o << "};\n    return t[c];\n}\n";
+    }
+    {
+        int empty_type = -1;
+        auto it = ctx.types.find("EMPTY");
+        if (it != ctx.types.end()) empty_type = it->second;
+        std::vector<bool> obst(n);
+        for (int i = 0; i < n; i++)
+            obst[i] = ctx.emission[i][0] == 0.0f && ctx.emission[i][1] == 0.0f && ctx.emission[i][2] == 0.0f &&
+                      ctx.mat_type[i] != empty_type;
+        o << "__device__ __forceinline__ bool se_light_obstacle(unsigned int c) { return " << em.bit(obst, "c")
+          << "; }\n";
+    }
     o << "__device__ __forceinline__ unsigned int se_lowbias(unsigned int h) {\n"
          "    h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;\n    return h;\n}\n";
     // guarded swap on register values (copies first: a conditional over lvalues

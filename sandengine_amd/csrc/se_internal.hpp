@@ -66,6 +66,7 @@ struct se_module {
     std::vector<unsigned> lds_bytes; // dynamic LDS per LDS variant
     std::vector<int> lds_grid;       // resident workgroups per LDS variant at full occupancy (multiple of 8)
     hipFunction_t plain = nullptr, fill = nullptr, fill_seeded = nullptr, copy = nullptr, fill_spec = nullptr;
+    hipFunction_t render = nullptr;  // se_render: colour and light side-outputs of a step
     hipFunction_t plain_v[6] = {};  // se_step_plain_r{1,2,4}_nt{0,1}
     std::vector<std::pair<se::LdsVariant, hipFunction_t>> lds;
     std::vector<hipFunction_t> batch;  // se_batch_lds_* parallel to `lds`
@@ -110,6 +111,10 @@ struct se_world {
     bool owns = true;       // buffers from hipMalloc (false: carved from a serving arena)
     int plain_variant = 3;  // index into se_module::plain_v (default: 1 row pair, non-temporal stores)
     int lds_variant = -1;   // index into se_module::lds, -1 = default choice
+    // render side-outputs (se_world_set_render): RGBA f32 per cell, row pitch `pitch`
+    void* light[2] = {nullptr, nullptr};  // ping-pong: light[lcur] holds the last step's light
+    void* color = nullptr;
+    int lcur = 0;
 };
 
 

@@ -819,3 +819,119 @@ extern "C" __global__ __launch_bounds__(256) void se_fill_seeded(u32* __restrict
         g[y * (size_t)pitch + x] = v;
     }
 }
+
+// ---------------------------------------------------------------------------
+// Render side-outputs of one step (setCell, shaders/compute/operations.glsl:99-170),
+// one thread per cell, after the step that turned `prev` into `cur`:
+//   colour: the cell's material colour; unless EMPTY, rgb minus
+//           noise(pos, 3, 2.0, 0.25) * 0.25, clamped to [0, 1] (math.glsl:80-106);
+//   light:  the material's emission if its rgb is non-zero, else full light on
+//           row 0, else the mix (0.5) of the average and the maximum of the 8
+//           neighbours' previous light (neighbours from the pre-step grid; a
+//           light obstacle passes only its falloff), in getDiagonalNeighbours
+//           order (math.glsl:154-166).
+// Float arithmetic follows the GLSL expression order with no contraction; sin
+// is this platform's sinf, so the noise matches the GL implementation's only
+// to within its sin error.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float se_fract(float x) { return x - floorf(x); }
+
+__device__ __forceinline__ float2 se_old_hash2(float px, float py) {
+    const float a = px * 127.1f + py * 311.7f, b = px * 269.5f + py * 183.3f;
+    return make_float2(-1.0f + 2.0f * se_fract(sinf(a) * 43758.5453123f),
+                       -1.0f + 2.0f * se_fract(sinf(b) * 43758.5453123f));
+}
+
+__device__ __forceinline__ float se_noise1(float px, float py) {
+    const float K1 = 0.366025404f, K2 = 0.211324865f;
+    const float s = (px + py) * K1;
+    const float ix = floorf(px + s), iy = floorf(py + s);
+    const float t = (ix + iy) * K2;
+    const float ax = px - ix + t, ay = py - iy + t;
+    const float m = ax < ay ? 0.0f : 1.0f;  // step(a.y, a.x)
+    const float ox = m, oy = 1.0f - m;
+    const float bx = ax - ox + K2, by = ay - oy + K2;
+    const float k2x2 = 2.0f * K2;
+    const float cx = ax - 1.0f + k2x2, cy = ay - 1.0f + k2x2;
+    const float h0 = fmaxf(0.5f - (ax * ax + ay * ay), 0.0f);
+    const float h1 = fmaxf(0.5f - (bx * bx + by * by), 0.0f);
+    const float h2 = fmaxf(0.5f - (cx * cx + cy * cy), 0.0f);
+    const float2 g0 = se_old_hash2(ix, iy), g1 = se_old_hash2(ix + ox, iy + oy), g2 = se_old_hash2(ix + 1.0f, iy + 1.0f);
+    const float n0 = h0 * h0 * h0 * h0 * (ax * g0.x + ay * g0.y);
+    const float n1 = h1 * h1 * h1 * h1 * (bx * g1.x + by * g1.y);
+    const float n2 = h2 * h2 * h2 * h2 * (cx * g2.x + cy * g2.y);
+    return 0.25f + 0.5f * (n0 * 70.0f + n1 * 70.0f + n2 * 70.0f);
+}
+
+__device__ __forceinline__ float se_noise(float px, float py) {  // noise(p, 3, 2.0, 0.25)
+    float f = 0.0f, qx = px, qy = py;
+#pragma unroll
+    for (int o = 1; o < 4; o++) {
+        f += 1.0f / (float)o * se_noise1(qx * 0.25f, qy * 0.25f);
+        qx *= 2.0f;
+        qy *= 2.0f;
+    }
+    return f;
+}
+
+extern "C" __global__ __launch_bounds__(256) void se_render(const u32* __restrict__ prev, const u32* __restrict__ cur,
+                                                            const float4* __restrict__ light_in,
+                                                            float4* __restrict__ light_out,
+                                                            float4* __restrict__ color_out, int W, int H, int pitch) {
+    const int x = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63), y = (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    if (x >= W || y >= H) return;
+    const size_t p = (size_t)y * pitch + x;
+    const u32 c = se_decode(cur[p]);
+    float4 col = se_color(c);
+    if (c != SE_EMPTY) {
+        const float r = se_noise((float)x, (float)y) * 0.25f;
+        col.x = fminf(fmaxf(col.x - r, 0.0f), 1.0f);
+        col.y = fminf(fmaxf(col.y - r, 0.0f), 1.0f);
+        col.z = fminf(fmaxf(col.z - r, 0.0f), 1.0f);
+    }
+    color_out[p] = col;
+    const float4 em = se_emission(c);
+    float4 light;
+    if (em.x != 0.0f || em.y != 0.0f || em.z != 0.0f) {
+        light = em;
+    } else if (y == 0) {
+        light = make_float4(1.0f, 1.0f, 1.0f, 0.999999f);
+    } else {
+        const int ndx[8] = {0, 0, -1, -1, 1, 1, 1, -1}, ndy[8] = {1, -1, 1, -1, 1, -1, 0, 0};
+        float4 avg = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mx = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float max_falloff = 0.0f;
+        int n = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int nx = x + ndx[k], ny = y + ndy[k];
+            if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
+            const size_t q = (size_t)ny * pitch + nx;
+            const float pass = se_light_obstacle(se_decode(prev[q])) ? 0.0f : 1.0f;
+            float4 ld = light_in[q];
+            ld.x *= pass;
+            ld.y *= pass;
+            ld.z *= pass;
+            const float falloff = ld.w == 0.0f ? max_falloff : ld.w;
+            const float4 l = make_float4(ld.x * ld.w, ld.y * ld.w, ld.z * ld.w, falloff);
+            avg.x += l.x;
+            avg.y += l.y;
+            avg.z += l.z;
+            avg.w += l.w;
+            max_falloff = fmaxf(falloff, max_falloff);
+            n++;
+            mx.x = fmaxf(mx.x, l.x);
+            mx.y = fmaxf(mx.y, l.y);
+            mx.z = fmaxf(mx.z, l.z);
+            mx.w = fmaxf(mx.w, l.w);
+        }
+        if (n > 0) {
+            const float fn = (float)n;
+            avg.x /= fn;
+            avg.y /= fn;
+            avg.z /= fn;
+            avg.w /= fn;
+        }
+        light = make_float4(avg.x * 0.5f + mx.x * 0.5f, avg.y * 0.5f + mx.y * 0.5f, avg.z * 0.5f + mx.z * 0.5f, avg.w);
+    }
+    light_out[p] = light;
+}

@@ -143,6 +143,7 @@ int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
     HIPCHECK(hipModuleGetFunction(&m->fill_seeded, m->mod, "se_fill_seeded"));
     HIPCHECK(hipModuleGetFunction(&m->copy, m->mod, "se_copy"));
     HIPCHECK(hipModuleGetFunction(&m->fill_spec, m->mod, "se_fill_spec"));
+    HIPCHECK(hipModuleGetFunction(&m->render, m->mod, "se_render"));
     int max_lds = 65536;
     hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
     if (info.table) {
@@ -214,6 +215,17 @@ int launch_fill(se_world* w, uint32_t* g, uint32_t v) {
     return launch(w->mod->fill, blocks, w->stream, g, n, v);
 }
 
+// Colour and light of the step that turned `prev` into `cur` (render enabled).
+int render_step(se_world* w, const uint32_t* prev, const uint32_t* cur) {
+    if (!w->color) return SE_OK;
+    dim3 grid((w->W + 63) / 64, (w->H + 3) / 4);
+    void* args[] = {(void*)&prev, (void*)&cur, &w->light[w->lcur], &w->light[w->lcur ^ 1], &w->color, (void*)&w->W,
+                    (void*)&w->H, (void*)&w->pitch};
+    HIPCHECK(hipModuleLaunchKernel(w->mod->render, grid.x, grid.y, 1, 256, 1, 1, 0, w->stream, args, nullptr));
+    w->lcur ^= 1;
+    return SE_OK;
+}
+
 }  // namespace
 
 // One Simulation::run with the plain kernel.
@@ -224,6 +236,7 @@ int se_rt::plain_step(se_world* w) {
     if (w->frame == 1) {  // falling_sand.glsl:128-131
         w->pending.clear();
         int rc = launch_fill(w, out, 0u);
+        if (!rc) rc = render_step(w, in, out);
         if (rc) return rc;
         w->cur ^= 1;
         return SE_OK;
@@ -242,6 +255,7 @@ int se_rt::plain_step(se_world* w) {
     int rc2 = launch(w->mod->plain_v[w->plain_variant], blocks, w->stream, (const uint32_t*)in, out, (int)w->W,
                      (int)w->H,
                      (int)w->pitch, (int)w->frame, ox, oy, strips, brows, (const void*)m.ptr, m.n);
+    if (!rc2) rc2 = render_step(w, in, out);
     if (rc2) return rc2;
     w->cur ^= 1;
     return SE_OK;
@@ -305,6 +319,13 @@ int lds_launch(se_world* w, const std::pair<se::LdsVariant, hipFunction_t>& v, i
 }
 
 int blocked_steps(se_world* w, uint32_t nsteps, uint32_t sub) {
+    if (w->color) {  // render side-outputs need every step's grid: one step per launch
+        for (uint32_t s = 0; s < nsteps; s++) {
+            int rc = plain_step(w);
+            if (rc) return rc;
+        }
+        return SE_OK;
+    }
     auto* v = pick_lds(w, sub);
     if (!v) return fail(SE_ERR_ARG, "no LDS variant supports " + std::to_string(sub) + " sub-steps");
     uint32_t k = sub ? sub : (uint32_t)v->first.hb;
@@ -503,6 +524,9 @@ void se_world_destroy(se_world* w) {
             hipFree(w->buf[1]);
             hipFree(w->d_mods);
         }
+        hipFree(w->light[0]);
+        hipFree(w->light[1]);
+        hipFree(w->color);
     }
     delete w;
 }
@@ -585,6 +609,57 @@ int se_variant_info(const se_world* w, int32_t kind, int32_t index, int32_t* a, 
     if (kind == 1) return put(v.tw, v.th, v.hb * 1000 + v.nt);
     if (kind == 3) return put(w->mod->lds_grid[index], (int32_t)w->mod->lds_bytes[index], 0);
     return fail(SE_ERR_ARG, "unknown variant kind");
+}
+
+int se_world_set_render(se_world* w, int enable) {
+    if (!w) return fail(SE_ERR_ARG, "null world");
+    DeviceGuard dg(w->ctx->device);
+    HIPCHECK(hipStreamSynchronize(w->stream));
+    if (!enable) {
+        hipFree(w->light[0]);
+        hipFree(w->light[1]);
+        hipFree(w->color);
+        w->light[0] = w->light[1] = w->color = nullptr;
+        return SE_OK;
+    }
+    if (w->color) return SE_OK;
+    const size_t bytes = (size_t)w->pitch * w->H * 16;
+    if (hipMalloc(&w->light[0], bytes) != hipSuccess || hipMalloc(&w->light[1], bytes) != hipSuccess ||
+        hipMalloc(&w->color, bytes) != hipSuccess) {
+        hipGetLastError();
+        hipFree(w->light[0]);
+        hipFree(w->light[1]);
+        hipFree(w->color);
+        w->light[0] = w->light[1] = w->color = nullptr;
+        return fail(SE_ERR_OOM, "hipMalloc of " + std::to_string(3 * bytes) + " bytes for render outputs failed");
+    }
+    for (void* p : {w->light[0], w->light[1], w->color}) HIPCHECK(hipMemsetAsync(p, 0, bytes, w->stream));
+    HIPCHECK(hipStreamSynchronize(w->stream));
+    w->lcur = 0;
+    return SE_OK;
+}
+
+int se_world_render_download(se_world* w, float* color, float* light) {
+    if (!w) return fail(SE_ERR_ARG, "null world");
+    if (!w->color) return fail(SE_ERR_STATE, "render outputs are not enabled for this world");
+    DeviceGuard dg(w->ctx->device);
+    if (color)
+        HIPCHECK(hipMemcpy2DAsync(color, (size_t)w->W * 16, w->color, (size_t)w->pitch * 16, (size_t)w->W * 16, w->H,
+                                  hipMemcpyDeviceToHost, w->stream));
+    if (light)
+        HIPCHECK(hipMemcpy2DAsync(light, (size_t)w->W * 16, w->light[w->lcur], (size_t)w->pitch * 16,
+                                  (size_t)w->W * 16, w->H, hipMemcpyDeviceToHost, w->stream));
+    HIPCHECK(hipStreamSynchronize(w->stream));
+    return SE_OK;
+}
+
+int se_world_render_ptrs(se_world* w, void** color, void** light, uint32_t* pitch) {
+    if (!w) return fail(SE_ERR_ARG, "null world");
+    if (!w->color) return fail(SE_ERR_STATE, "render outputs are not enabled for this world");
+    if (color) *color = w->color;
+    if (light) *light = w->light[w->lcur];
+    if (pitch) *pitch = w->pitch;
+    return SE_OK;
 }
 
 // ------------------------------------------------------------------ save / load
