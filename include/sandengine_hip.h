@@ -181,18 +181,22 @@ void se_batch_destroy(se_batch* b);
 int se_batch_add(se_batch* b, se_world* w, uint32_t nsteps);
 int se_batch_clear(se_batch* b);
 /* mode 0: loop the plain kernel world by world (one launch per step);
- * mode 1: batched persistent LDS kernel -- worlds that fit one tile advance
+ * modes 1-3: the batched LDS kernel -- worlds that fit one tile advance
  * through all their steps in one work item, larger ones in rounds of k
  * sub-steps, rule-set groups and the two kinds overlapping on separate streams.
+ * Its launch shape: mode 2 one workgroup per work item, mode 3 persistent (as
+ * many workgroups as are resident, striding over the items), mode 1 the
+ * autotuned shape (else one workgroup per item).
  * lds_variant -1 / k 0: the autotuned (else default) choice; graph != 0
  * captures the whole submission in a hipGraph and launches that.
  * *gpu_ms: time between events around the submission. */
 int se_batch_run(se_batch* b, int mode, int32_t lds_variant, uint32_t k, int graph, float* gpu_ms);
 /* Autotuner: for each rule-set group, times every LDS variant with k = halo and
- * k = halo/2 on `rounds` rounds of the batch's own worlds (grids and frames are
- * snapshotted and restored) and keeps the fastest for later runs. */
+ * k = halo/2, in both launch shapes, on `rounds` rounds of the batch's own
+ * worlds (grids and frames are snapshotted and restored) and keeps the fastest
+ * for later runs.  The outputs describe the last group tuned. */
 int se_batch_autotune(se_batch* b, uint32_t rounds, int32_t* best_variant, uint32_t* best_k,
-                      float* best_ns_per_kcell);
+                      int32_t* best_persistent, float* best_ns_per_kcell);
 
 /* ---------------------------------------------------------------- serving loop
  * No reference counterpart (the reference steps one interactive world per

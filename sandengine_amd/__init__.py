@@ -128,7 +128,7 @@ def lib():
         L.se_batch_clear.argtypes = [c_p]
         L.se_batch_run.argtypes = [c_p, ctypes.c_int, c_i32, c_u32, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
         L.se_batch_autotune.argtypes = [c_p, c_u32, ctypes.POINTER(c_i32), ctypes.POINTER(c_u32),
-                                        ctypes.POINTER(ctypes.c_float)]
+                                        ctypes.POINTER(c_i32), ctypes.POINTER(ctypes.c_float)]
         L.se_variant_info.argtypes = [c_p, c_i32, c_i32] + [ctypes.POINTER(c_i32)] * 3
         _lib = L
     return _lib
@@ -518,8 +518,9 @@ class Serve:
 
 
 class Batch:
-    """Many worlds advanced together (se_batch): looped plain kernel or the
-    batched persistent LDS kernel, optionally through one hipGraph."""
+    """Many worlds advanced together (se_batch): looped plain kernel (mode 0) or
+    the batched LDS kernel, launched one workgroup per work item (mode 2),
+    persistently (mode 3) or as tuned (mode 1), optionally through one hipGraph."""
 
     def __init__(self, ctx: Context):
         p = ctypes.c_void_p()
@@ -539,10 +540,12 @@ class Batch:
         _check(lib().se_batch_run(self._p, mode, variant, k, 1 if graph else 0, ctypes.byref(ms)))
         return ms.value
 
-    def autotune(self, rounds: int = 3) -> Tuple[int, int, float]:
-        v, k, s = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_float()
-        _check(lib().se_batch_autotune(self._p, rounds, ctypes.byref(v), ctypes.byref(k), ctypes.byref(s)))
-        return v.value, k.value, s.value
+    def autotune(self, rounds: int = 3) -> Tuple[int, int, bool, float]:
+        """(LDS variant, k, persistent launch, ns per 1000 cell-updates) of the last group tuned."""
+        v, k, p, s = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_int32(), ctypes.c_float()
+        _check(lib().se_batch_autotune(self._p, rounds, ctypes.byref(v), ctypes.byref(k), ctypes.byref(p),
+                                       ctypes.byref(s)))
+        return v.value, k.value, bool(p.value), s.value
 
     def close(self):
         if getattr(self, "_p", None) and self._p.value:

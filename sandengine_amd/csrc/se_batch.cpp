@@ -48,7 +48,12 @@ struct DevArray {
 struct se_batch {
     se_ctx* ctx = nullptr;
     std::vector<std::pair<se_world*, uint32_t>> items;
-    std::map<se_module*, std::pair<int, uint32_t>> tuned;  // module -> (LDS variant index, k)
+    struct Tuned {
+        int vi;           // LDS variant index
+        uint32_t k;       // sub-steps per round
+        bool persistent;  // launch shape (see enqueue_plan)
+    };
+    std::map<se_module*, Tuned> tuned;
     hipStream_t streams[8] = {};
     ~se_batch() {
         for (auto s : streams)
@@ -66,6 +71,7 @@ struct Plan {
     std::vector<int> n_active;                                  // per round, large list
     DevArray d_small, d_large, d_pre_small, d_pre_large;
     std::vector<int> pre_small, pre_large;                      // host copies of the tile prefix sums
+    bool persistent = false;                                     // launch shape (see enqueue_plan)
 };
 
 bool fits_one_tile(const se_world* w, const se::LdsVariant& v) {
@@ -133,25 +139,32 @@ int make_plan(se_ctx* ctx, se_module* mod, int vi, uint32_t k, const std::vector
 }
 
 // Enqueues a plan's launches: single-tile worlds on `s_small`, rounds on `s_large`.
-// One workgroup per work item (the kernel's item loop then runs once): workgroups
-// that start as others retire keep loads and compute of different items
-// overlapped, and the longest-first order makes the dispatch order a
-// longest-first schedule.
+// Two launch shapes of the same kernel:
+//   * one workgroup per work item (the kernel's item loop runs once);
+//     workgroups that start as others retire keep the loads and compute of
+//     different items overlapped, and the longest-first order makes the
+//     dispatch order a longest-first schedule;
+//   * persistent: as many workgroups as are resident at once, each striding
+//     over the items.
+// The autotuner measures both.
 int enqueue_plan(Plan& P, hipStream_t s_small, hipStream_t s_large) {
     const se::LdsVariant& v = P.mod->lds[P.vi].first;
     hipFunction_t f = P.mod->batch[P.vi];
     const unsigned shmem = P.mod->lds_bytes[P.vi];
     const uint32_t* tbl = P.mod->table;
+    auto grid = [&](int items) {
+        return (unsigned)(P.persistent ? std::min(items, P.mod->lds_grid[P.vi]) : items);
+    };
     if (!P.small.empty()) {
         int n = (int)P.small.size();
-        int rc = launch_ex(f, (unsigned)P.pre_small.back(), (unsigned)v.nt, shmem, s_small, (const void*)P.d_small.p,
+        int rc = launch_ex(f, grid(P.pre_small.back()), (unsigned)v.nt, shmem, s_small, (const void*)P.d_small.p,
                            (const int*)P.d_pre_small.p, n, 0, (int)(1 << 30), tbl);
         if (rc) return rc;
     }
     for (size_t r = 0; r < P.n_active.size(); r++) {
-        int rc = launch_ex(f, (unsigned)P.pre_large[P.n_active[r]], (unsigned)v.nt, shmem, s_large,
-                           (const void*)P.d_large.p,
-                           (const int*)P.d_pre_large.p, P.n_active[r], (int)r, (int)P.k, tbl);
+        int rc = launch_ex(f, grid(P.pre_large[P.n_active[r]]), (unsigned)v.nt, shmem, s_large,
+                           (const void*)P.d_large.p, (const int*)P.d_pre_large.p, P.n_active[r], (int)r, (int)P.k,
+                           tbl);
         if (rc) return rc;
     }
     return SE_OK;
@@ -177,8 +190,9 @@ int ensure_streams(se_batch* b, int n) {
 
 // Runs the batch: plans are built for every rule-set group, then all launches
 // are submitted (optionally through a hipGraph) between two events.
+// launch: 0 one workgroup per item, 1 persistent, -1 the tuned choice (else one per item)
 int run_batched(se_batch* b, const std::vector<std::pair<se_world*, uint32_t>>& work, int32_t variant, uint32_t k,
-                int graph, float* gpu_ms) {
+                int launch, int graph, float* gpu_ms) {
     se_ctx* ctx = b->ctx;
     std::map<se_module*, std::vector<std::pair<se_world*, uint32_t>>> groups;
     for (auto& it : work) groups[it.first->mod.get()].push_back(it);
@@ -187,17 +201,20 @@ int run_batched(se_batch* b, const std::vector<std::pair<se_world*, uint32_t>>& 
         se_module* mod = g.first;
         int vi = variant;
         uint32_t kk = k;
+        int la = launch;
+        auto t = b->tuned.find(mod);
         if (vi < 0) {
-            auto t = b->tuned.find(mod);
             if (t != b->tuned.end()) {
-                vi = t->second.first;
-                if (kk == 0) kk = t->second.second;
+                vi = t->second.vi;
+                if (kk == 0) kk = t->second.k;
             } else {
                 vi = se_rt::default_lds_index(mod);
             }
         }
+        if (la < 0) la = t != b->tuned.end() ? (int)t->second.persistent : 0;
         if (vi < 0 || vi >= (int)mod->lds.size()) return fail(SE_ERR_ARG, "LDS variant out of range");
         plans.emplace_back(new Plan());
+        plans.back()->persistent = la == 1;
         int rc = make_plan(ctx, mod, vi, kk, g.second, *plans.back());
         if (rc) return rc;
     }
@@ -329,15 +346,17 @@ int se_batch_run(se_batch* b, int mode, int32_t lds_variant, uint32_t k, int gra
         if (gpu_ms) *gpu_ms = ms;
         return SE_OK;
     }
-    if (mode != 1) return fail(SE_ERR_ARG, "mode must be 0 (looped plain) or 1 (batched)");
+    if (mode < 1 || mode > 3)
+        return fail(SE_ERR_ARG, "mode must be 0 (looped plain), 1 (batched, tuned launch), 2 (batched, one "
+                                "workgroup per item) or 3 (batched, persistent)");
     int rc = pre_steps(work);
     if (rc) return rc;
     HIPCHECK(hipStreamSynchronize(b->ctx->stream));
-    return run_batched(b, work, lds_variant, k, graph, gpu_ms);
+    return run_batched(b, work, lds_variant, k, mode == 1 ? -1 : mode - 2, graph, gpu_ms);
 }
 
 int se_batch_autotune(se_batch* b, uint32_t rounds, int32_t* best_variant, uint32_t* best_k,
-                      float* best_ns_per_kcell) {
+                      int32_t* best_persistent, float* best_ns_per_kcell) {
     if (!b || rounds == 0) return fail(SE_ERR_ARG, "bad arguments");
     DeviceGuard dg(b->ctx->device);
     int rc = SE_OK;
@@ -368,9 +387,11 @@ int se_batch_autotune(se_batch* b, uint32_t rounds, int32_t* best_variant, uint3
         double best = 1e300;
         int bv = 0;
         uint32_t bk = 0;
+        bool bp = false;
         for (int vi = 0; vi < (int)mod->lds.size(); vi++) {
             const int hb = mod->lds[vi].first.hb;
-            for (uint32_t kk : {(uint32_t)hb, (uint32_t)std::max(2, hb / 2)}) {
+            for (uint32_t kk : {(uint32_t)hb, (uint32_t)std::max(2, hb / 2)})
+            for (int la = 0; la < 2; la++) {
                 std::vector<std::pair<se_world*, uint32_t>> trial;
                 double cells = 0;
                 for (auto& it : g.second) {
@@ -379,7 +400,7 @@ int se_batch_autotune(se_batch* b, uint32_t rounds, int32_t* best_variant, uint3
                     cells += (double)it.first->W * it.first->H * st;
                 }
                 float ms = 0;
-                rc = run_batched(b, trial, vi, kk, 0, &ms);
+                rc = run_batched(b, trial, vi, kk, la, 0, &ms);
                 // restore
                 for (size_t i = 0; i < g.second.size(); i++) {
                     se_world* w = g.second[i].first;
@@ -398,13 +419,15 @@ int se_batch_autotune(se_batch* b, uint32_t rounds, int32_t* best_variant, uint3
                     best = score;
                     bv = vi;
                     bk = kk;
+                    bp = la == 1;
                 }
             }
         }
         for (auto& s : snaps) (void)hipFree(s.first);
-        b->tuned[mod] = {bv, bk};
+        b->tuned[mod] = {bv, bk, bp};
         if (best_variant) *best_variant = bv;
         if (best_k) *best_k = bk;
+        if (best_persistent) *best_persistent = bp ? 1 : 0;
         if (best_ns_per_kcell) *best_ns_per_kcell = (float)best;
     }
     return SE_OK;

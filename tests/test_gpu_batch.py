@@ -82,12 +82,14 @@ def test_batch_modes_and_autotune_agree():
     b = S.Batch(ctx())
     for w, sp in zip(worlds, specs):
         b.add(w, sp[3])
-    v, k, score = b.autotune(rounds=2)
+    v, k, persistent, score = b.autotune(rounds=2)
     assert k >= 2 and score > 0
     b.run(mode=1)
     b.run(mode=0)
+    b.run(mode=2, graph=True)
+    b.run(mode=3)
     for w, t, sp in zip(worlds, twins, specs):
-        t.step(2 * sp[3])
+        t.step(4 * sp[3])
         np.testing.assert_array_equal(w.download(), t.download(), err_msg=str(sp))
     b.close()
 
