@@ -843,7 +843,7 @@ std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info) {
         }
         uint64_t tuples = (uint64_t)n * n * n * n;
         uint64_t bytes = tuples * R * 4;
-        bool ok = !em.rand_float_used && n <= 127 && bytes <= kTableMaxBytes;
+        bool ok = !em.rand_float_used && bytes <= kTableMaxBytes;
         o << "#define SE_TBL " << (ok ? 1 : 0) << "\n";
         o << "#define SE_TBL_R " << R << "u\n";
         o << "#define SE_TBL_ENTRIES " << (ok ? tuples * R : 1) << "u\n";

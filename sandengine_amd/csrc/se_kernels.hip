@@ -267,7 +267,6 @@ extern "C" __global__ __launch_bounds__(256) void se_build_table(u32* __restrict
     t /= SE_NMAT;
     const u32 i2 = t % SE_NMAT;
     const u32 i3 = t / SE_NMAT;
-    u32 first = 0, same = 1;
     for (u32 r = 0; r < SE_TBL_R; r++) {
         u32 h[4], rr = r;
 #pragma unroll
@@ -278,13 +277,8 @@ extern "C" __global__ __launch_bounds__(256) void se_build_table(u32* __restrict
         }
         u32 c0 = i0, c1 = i1, c2 = i2, c3 = i3;
         se_block_h(c0, c1, c2, c3, h[0], h[1], h[2], h[3]);
-        const u32 e = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
-        if (r == 0) first = e;
-        same &= (u32)(e == first);
-        tbl[tuple * SE_TBL_R + r] = e;
+        tbl[tuple * SE_TBL_R + r] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
     }
-    // bit 31 of the class-0 entry: the result does not depend on the hash (ids < 128)
-    if (same) tbl[tuple * SE_TBL_R] = first | 0x80000000u;
 }
 
 // ---------------------------------------------------------------------------
@@ -385,26 +379,19 @@ __device__ __forceinline__ void se_hash_lanes(u32* h, const u32* xa, const u32* 
 }
 
 // Table path: new packed blocks for two packed blocks ta (world row gya) and tb
-// (row gyb).  Both class-0 entries are read at once; the hashes are computed
-// (for both) only where one of the two results depends on the hash classes.
-// xa[l] + gy * kb(l) + kf[l] is the hash input of lane l.
+// (row gyb): one table read each, at the tuple's entry for the block's hash
+// class.  Hashing every block and reading once measured faster than reading a
+// hash-independence flag first and hashing only where needed (a second
+// dependent LDS read and a branch per block).  xa[l] + gy * kb(l) + kf[l] is the
+// hash input of lane l.
 __device__ __forceinline__ void se_blk_eval2(const u32* __restrict__ ltbl, u32 ta, u32 tb, const u32* xa,
                                              const u32* kf, int gya, int gyb, u32& ea, u32& eb) {
 #if SE_TBL
-    const u32 ba = se_tuple(ta) * SE_TBL_R, bb = se_tuple(tb) * SE_TBL_R;
-    ea = ltbl[ba];
-    eb = ltbl[bb];
-    if (!((ea & eb) >> 31)) {
-        u32 ha[4], hb[4];
-        se_hash_lanes(ha, xa, kf, gya);
-        se_hash_lanes(hb, xa, kf, gyb);
-        const u32 ra = (ea >> 31) ? 0u : se_rclass(ha[0], ha[1], ha[2], ha[3]);
-        const u32 rb = (eb >> 31) ? 0u : se_rclass(hb[0], hb[1], hb[2], hb[3]);
-        ea = ltbl[ba + ra];
-        eb = ltbl[bb + rb];
-    }
-    ea &= 0x7fffffffu;
-    eb &= 0x7fffffffu;
+    u32 ha[4], hb[4];
+    se_hash_lanes(ha, xa, kf, gya);
+    se_hash_lanes(hb, xa, kf, gyb);
+    ea = ltbl[se_tuple(ta) * SE_TBL_R + se_rclass(ha[0], ha[1], ha[2], ha[3])];
+    eb = ltbl[se_tuple(tb) * SE_TBL_R + se_rclass(hb[0], hb[1], hb[2], hb[3])];
 #else
     (void)ltbl; (void)ta; (void)tb; (void)xa; (void)kf; (void)gya; (void)gyb;
     ea = eb = 0u;
@@ -586,13 +573,10 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                 const int gx = gx0 + lx, gy = gy0 + ly;
 #if SE_TBL
                 const u32 base = (c0 + SE_NMAT * (c1 + SE_NMAT * (c2 + SE_NMAT * c3))) * SE_TBL_R;
-                u32 e = ltbl[base];
-                if (!(e >> 31)) {  // the result depends on the block's hash classes
-                    const u32 r = se_rclass(se_hash_lane(0, (u32)gx, (u32)gy, f), se_hash_lane(1, (u32)gx, (u32)gy, f),
-                                            se_hash_lane(2, (u32)gx, (u32)gy, f), se_hash_lane(3, (u32)gx, (u32)gy, f));
-                    e = ltbl[base + r];
-                }
-                c0 = e & 0xffu; c1 = (e >> 8) & 0xffu; c2 = (e >> 16) & 0xffu; c3 = (e >> 24) & 0x7fu;
+                const u32 r = se_rclass(se_hash_lane(0, (u32)gx, (u32)gy, f), se_hash_lane(1, (u32)gx, (u32)gy, f),
+                                        se_hash_lane(2, (u32)gx, (u32)gy, f), se_hash_lane(3, (u32)gx, (u32)gy, f));
+                const u32 e = ltbl[base + r];
+                c0 = e & 0xffu; c1 = (e >> 8) & 0xffu; c2 = (e >> 16) & 0xffu; c3 = e >> 24;
 #else
                 se_block(c0, c1, c2, c3, (u32)gx, (u32)gy, f);
 #endif
