@@ -248,17 +248,22 @@ print(json.dumps(dict(value=side * side * steps / dt / 1e9, unit='Gcell-updates/
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def load_traffic(kernel_prefix):
-    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
-    scaled by the copy-kernel calibration), from profiles/traffic.json."""
+def load_profile(kernel_prefix):
+    """Per-launch figures rocprofv3 measured for a kernel (profiles/traffic.json):
+    HBM bytes (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and
+    VALU / LDS wavefront instructions."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
-        return None
+        return {}
     d = json.load(open(p))
     for k, v in d.get("kernels", {}).items():
         if k.startswith(kernel_prefix):
-            return v.get("bytes_per_launch")
-    return None
+            return v
+    return {}
+
+
+# One wave64 VALU instruction per cycle per CU (4 SIMD16 units), 256 CUs, 2.4 GHz.
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 2.4e9
 
 
 def main():
@@ -346,7 +351,15 @@ def main():
     best = max(results, key=lambda m: results[m]["gcups"])
     R = results[best]
     kernel = "se_step_plain_r1_nt1" if best == "plain" else lds_kernel
-    traffic = load_traffic(kernel)
+    prof = load_profile(kernel)
+    traffic = prof.get("bytes_per_launch")
+    valu = None
+    if prof.get("valu_wave_instr_per_launch"):
+        rate = prof["valu_wave_instr_per_launch"] / (R["avg_launch_ms"] / 1e3)
+        valu = {"bound": "valu", "kernel": kernel, "achieved": rate, "peak": VALU_PEAK_WAVE_INSTR_PER_S,
+                "unit": "wave64 VALU instructions/s", "frac": rate / VALU_PEAK_WAVE_INSTR_PER_S,
+                "instr_per_launch": prof["valu_wave_instr_per_launch"],
+                "source": "SQ_INSTS_VALU per launch from profiles/traffic.json over this run's launch time"}
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload)
     out = {
         "metric": "Gcell-updates/s & HBM-roofline %; p50/p99 step latency, batched vs plain",
@@ -365,7 +378,9 @@ def main():
                    "steps_per_launch": R["steps_per_launch"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": R["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": R["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": 8 * cells, "avg_launch_ms": R["avg_launch_ms"]},
+                     "algorithmic_bytes_per_launch": 8 * cells, "avg_launch_ms": R["avg_launch_ms"],
+                     "binding_limit": "hbm" if best == "plain" else "valu (see roofline_valu)"},
+        "roofline_valu": valu,
         "cpu_baseline": cpu,
         "copy_reference": {"gbs": copy_gbs, "ms": copy_ms, "frac_of_peak": copy_gbs / HBM_PEAK_GBS},
         "speedup_blocked_vs_plain": (results["blocked"]["gcups"] / results["plain"]["gcups"]
