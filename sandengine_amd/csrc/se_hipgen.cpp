@@ -20,6 +20,7 @@ static const char* kKernelTemplates =
 namespace {
 
 constexpr uint64_t kTableMaxBytes = 48 * 1024;  // transition table staged in LDS
+constexpr uint64_t kCTableMaxBytes = 64 * 1024;  // coded table: D^4 u8 entries staged in LDS
 
 [[noreturn]] void gen_error(const std::string& m) { throw LangError("NotRecognized", "(HIP codegen) " + m); }
 
@@ -681,7 +682,7 @@ std::string emit_rule(const SandRule& r, const Ctx& ctx, Emitter& em) {
 
 }  // namespace
 
-std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info) {
+std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info, bool allow_ctable) {
     Ctx ctx = build_ctx(r);
     Emitter em(ctx);
     std::string mirrored, right;
@@ -844,9 +845,34 @@ std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info) {
         uint64_t tuples = (uint64_t)n * n * n * n;
         uint64_t bytes = tuples * R * 4;
         bool ok = !em.rand_float_used && bytes <= kTableMaxBytes;
-        o << "#define SE_TBL " << (ok ? 1 : 0) << "\n";
+        // Coded table, for rule sets whose full table does not fit: tuples of the
+        // D = n - 2 materials other than NULL and WALL (blocks holding those run
+        // the chain), the mirror factored out as a byte permutation (lane 0 only
+        // decides the mirror and there are no Right rules), and per tuple a u8
+        // naming its outcome vector over the classes of lanes 1..3.  Whether the
+        // rule set fits that form is checked on the enumerated table when the rule
+        // set is loaded; it is generated without it otherwise.
+        const uint64_t D = (uint64_t)n - 2, Rc = R / 2;
+        const uint64_t ct_bytes = D * D * D * D, t1_words = (ct_bytes + 15) / 16 * 4;
+        const bool ct = allow_ctable && !ok && !em.rand_float_used && right.empty() && th[0].size() == 1 &&
+                        n >= 3 && D <= 127 && ct_bytes <= kCTableMaxBytes;
+        o << "#define SE_TBL " << (ok ? 1 : ct ? 2 : 0) << "\n";
         o << "#define SE_TBL_R " << R << "u\n";
         o << "#define SE_TBL_ENTRIES " << (ok ? tuples * R : 1) << "u\n";
+        if (ct) {
+            o << "#define SE_CT_D " << D << "u\n";
+            o << "#define SE_CT_R " << Rc << "u\n";
+            o << "#define SE_CT_T1_WORDS " << t1_words << "u\n";
+            o << "#define SE_CT_MIRROR_T " << hex_u32((uint64_t)tmirror) << "\n";
+            o << "__device__ __forceinline__ unsigned int se_rclass_ct(unsigned int h1, unsigned int h2, "
+                 "unsigned int h3) {\n    (void)h1; (void)h2; (void)h3;\n    unsigned int r = 0u;\n";
+            uint64_t st = 1;
+            for (int l = 1; l < 4; l++) {
+                for (uint32_t t : th[l]) o << "    r += (h" << l << " > " << hex_u32(t) << ") ? " << st << "u : 0u;\n";
+                st *= th[l].size() + 1;
+            }
+            o << "    return r;\n}\n";
+        }
         // class index: lane l contributes (#thresholds below h) * stride_l
         o << "__device__ __forceinline__ unsigned int se_rclass(unsigned int h0, unsigned int h1, unsigned int h2, "
              "unsigned int h3) {\n    (void)h0; (void)h1; (void)h2; (void)h3;\n    unsigned int r = 0u;\n";
@@ -871,6 +897,9 @@ std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info) {
         if (info) {
             info->table = ok;
             info->table_R = (int)R;
+            info->ctable = ct;
+            info->ct_D = ct ? (int)D : 0;
+            info->ct_R = ct ? (int)Rc : 0;
         }
     }
     if (info) {
@@ -886,8 +915,8 @@ std::string lds_kernel_name(const LdsVariant& v) {
            std::to_string(v.nt);
 }
 
-std::string hip_program_source(const ParsingResult& r, const std::vector<LdsVariant>& lds) {
-    std::string s = hip_rules_source(r);
+std::string hip_program_source(const ParsingResult& r, const std::vector<LdsVariant>& lds, bool allow_ctable) {
+    std::string s = hip_rules_source(r, nullptr, allow_ctable);
     s += "\n";
     s += kKernelTemplates;
     s += "\n";

@@ -29,13 +29,16 @@ struct HipGenInfo {
     int nrules = 0;
     bool table = false;  // transition table mode (fits LDS, thresholds only)
     int table_R = 1;     // rand classes per 4-id tuple
+    bool ctable = false; // coded-table candidate (SE_TBL 2): fit checked when the rule set is loaded
+    int ct_D = 0;        // materials the coded table covers (all but NULL and WALL)
+    int ct_R = 0;        // classes of hash lanes 1..3
 };
 
 // Generated device code for the rule set (SE_NMAT, se_fixed, se_block, ...).
-std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info = nullptr);
+std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info = nullptr, bool allow_ctable = true);
 
 // Full translation unit: rules + kernel templates + LDS instantiations.
-std::string hip_program_source(const ParsingResult& r, const std::vector<LdsVariant>& lds);
+std::string hip_program_source(const ParsingResult& r, const std::vector<LdsVariant>& lds, bool allow_ctable = true);
 
 // Name of an LDS kernel instantiation.
 std::string lds_kernel_name(const LdsVariant& v);

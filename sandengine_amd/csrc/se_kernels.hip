@@ -252,6 +252,24 @@ extern "C" __global__ __launch_bounds__(256) void se_fill(u32* __restrict__ g, s
 }
 
 // ---------------------------------------------------------------------------
+// Cell codes in LDS.  With the coded table (SE_TBL == 2) the LDS tile holds
+// code(id) = (id - 3) mod SE_NMAT: the rule set's materials 3.. become 0..,
+// EMPTY becomes D - 1, NULL D and WALL D + 1 (D = SE_NMAT - 2), so the tuples the
+// table covers are those of codes < D.  Every other path keeps ids in LDS.
+// se_enc_ok takes an id < SE_NMAT; se_enc decodes any u32 first.
+// ---------------------------------------------------------------------------
+#if SE_TBL == 2
+#define SE_LWALL (SE_NMAT - 1u)
+__device__ __forceinline__ u32 se_enc_ok(u32 id) { return min(id - 3u, id + (SE_NMAT - 3u)); }
+__device__ __forceinline__ u32 se_dec(u32 c) { return min(c + 3u, c - (SE_NMAT - 3u)); }
+#else
+#define SE_LWALL SE_WALLID
+__device__ __forceinline__ u32 se_enc_ok(u32 id) { return id; }
+__device__ __forceinline__ u32 se_dec(u32 c) { return c; }
+#endif
+__device__ __forceinline__ u32 se_enc(u32 v) { return se_enc_ok(se_decode(v)); }
+
+// ---------------------------------------------------------------------------
 // Transition table (SE_TBL == 1): entry tuple * SE_TBL_R + r holds the block
 // result (four ids packed as bytes c0 | c1<<8 | c2<<16 | c3<<24) for ids
 // tuple = c0 + N*(c1 + N*(c2 + N*c3)) and hash-threshold class r.  Built on the
@@ -281,6 +299,40 @@ extern "C" __global__ __launch_bounds__(256) void se_build_table(u32* __restrict
     }
 }
 
+#if SE_TBL == 2
+// Coded table, raw form for the host, which names the distinct outcomes and
+// packs the LDS table: for each tuple of codes < D (index c0 + D*(c1 + D*(c2 +
+// D*c3))), each class r of hash lanes 1..3 and each mirror bit m, the block
+// result as ids packed in bytes, from se_block_h itself with one representative
+// hash per class (lane 0 at the mirror threshold for m = 1, above it for m = 0).
+extern "C" __global__ __launch_bounds__(256) void se_build_ctable(u32* __restrict__ raw) {
+    const u32 tuple = blockIdx.x * 256u + threadIdx.x;
+    if (tuple >= SE_CT_D * SE_CT_D * SE_CT_D * SE_CT_D) return;
+    u32 t = tuple;
+    const u32 i0 = se_dec(t % SE_CT_D);
+    t /= SE_CT_D;
+    const u32 i1 = se_dec(t % SE_CT_D);
+    t /= SE_CT_D;
+    const u32 i2 = se_dec(t % SE_CT_D);
+    const u32 i3 = se_dec(t / SE_CT_D);
+    for (u32 r = 0; r < SE_CT_R; r++) {
+        u32 h[4], rr = r;
+#pragma unroll
+        for (int l = 1; l < 4; l++) {
+            const u32 cnt = se_rclass_count(l);
+            h[l] = se_rclass_rep(l, rr % cnt);
+            rr /= cnt;
+        }
+        for (u32 m = 0; m < 2; m++) {
+            h[0] = m ? SE_CT_MIRROR_T : 0xffffffffu;
+            u32 c0 = i0, c1 = i1, c2 = i2, c3 = i3;
+            se_block_h(c0, c1, c2, c3, h[0], h[1], h[2], h[3]);
+            raw[(tuple * SE_CT_R + r) * 2u + m] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
+        }
+    }
+}
+#endif
+
 // ---------------------------------------------------------------------------
 // LDS temporal-blocked kernel: a workgroup of NT threads stages a
 // (TW + 2*HB) x (TH + 2*HB) tile (u8 ids) in LDS and advances it `nsub` <= HB
@@ -305,9 +357,13 @@ __device__ __forceinline__ u32 se_pack4(u32 a, u32 b, u32 c, u32 d) { return a |
 // test for the quad; the per-cell decode runs only where one is out of range).
 __device__ __forceinline__ u32 se_pack_quad(uint4 v) {
     const u32 mx = max(max(v.x, v.y), max(v.z, v.w));
-    if (__builtin_expect(mx >= SE_NMAT, 0))
-        return se_pack4(se_decode(v.x), se_decode(v.y), se_decode(v.z), se_decode(v.w));
-    return se_pack4(v.x, v.y, v.z, v.w);
+    if (__builtin_expect(mx >= SE_NMAT, 0)) return se_pack4(se_enc(v.x), se_enc(v.y), se_enc(v.z), se_enc(v.w));
+    return se_pack4(se_enc_ok(v.x), se_enc_ok(v.y), se_enc_ok(v.z), se_enc_ok(v.w));
+}
+
+// Four LDS bytes as HBM ids.
+__device__ __forceinline__ uint4 se_unpack_quad(u32 p) {
+    return make_uint4(se_dec(p & 0xffu), se_dec((p >> 8) & 0xffu), se_dec((p >> 16) & 0xffu), se_dec(p >> 24));
 }
 
 // Block-origin range along one axis at sub-step s: origins o (local) with parity
@@ -328,20 +384,16 @@ __device__ __forceinline__ void se_block_range(int s, int par, int g0, int L, in
     cnt = b >= a ? (b - a) / 2 + 1 : 0;
 }
 
-#if SE_TBL
 // Tuple index c0 + N*c1 + N^2*c2 + N^3*c3 of a packed block t = c0 | c1<<8 |
-// c2<<16 | c3<<24 (ids < N): byte dot products with constant weights.
+// c2<<16 | c3<<24 (bytes < N <= 255): byte dot products with constant weights.
+template <u32 N>
 __device__ __forceinline__ u32 se_tuple(u32 t) {
-#if SE_NMAT <= 6
-    return __builtin_amdgcn_udot4(t, 1u | (SE_NMAT << 8) | ((SE_NMAT * SE_NMAT) << 16) |
-                                         ((SE_NMAT * SE_NMAT * SE_NMAT) << 24), 0u, false);
-#else
-    const u32 lo = __builtin_amdgcn_udot4(t, 1u | (SE_NMAT << 8), 0u, false);
-    const u32 hi = __builtin_amdgcn_udot4(t, (1u << 16) | (SE_NMAT << 24), 0u, false);
-    return lo + __umul24(hi, SE_NMAT * SE_NMAT);
-#endif
+    if (N <= 6) return __builtin_amdgcn_udot4(t, 1u | (N << 8) | ((N * N) << 16) | ((N * N * N) << 24), 0u, false);
+    if (N <= 15) return __builtin_amdgcn_udot4(t, 1u | (N << 8) | ((N * N) << 16), __umul24(t >> 24, N * N * N), false);
+    const u32 lo = __builtin_amdgcn_udot4(t, 1u | (N << 8), 0u, false);
+    const u32 hi = __builtin_amdgcn_udot4(t, (1u << 16) | (N << 24), 0u, false);
+    return lo + __umul24(hi, N * N);
 }
-#endif
 
 // A block's four cells from LDS as c0 | c1<<8 | c2<<16 | c3<<24: two 16-bit
 // accesses at even x, four byte accesses at odd x.
@@ -370,7 +422,7 @@ __device__ __forceinline__ void se_blk_write(unsigned char* p0, u32 e) {
 __device__ __forceinline__ u32 se_keep_wall(u32 e, u32 colkeep, int gy, int H) {
     const u32 rowkeep = ((gy >= 0 && gy < H) ? 0x0000ffffu : 0u) | ((gy + 1 >= 0 && gy + 1 < H) ? 0xffff0000u : 0u);
     const u32 keep = colkeep & rowkeep;
-    return (e & keep) | (0x02020202u & ~keep);
+    return (e & keep) | ((SE_LWALL * 0x01010101u) & ~keep);
 }
 
 __device__ __forceinline__ void se_hash_lanes(u32* h, const u32* xa, const u32* kf, int gy) {
@@ -384,14 +436,54 @@ __device__ __forceinline__ void se_hash_lanes(u32* h, const u32* xa, const u32* 
 // hash-independence flag first and hashing only where needed (a second
 // dependent LDS read and a branch per block).  xa[l] + gy * kb(l) + kf[l] is the
 // hash input of lane l.
+// Does a packed block or quad hold a code >= D (NULL or WALL with the coded table)?
+__device__ __forceinline__ bool se_has_fixed(u32 t) {
+#if SE_TBL == 2
+    return ((t + (0x80u - SE_CT_D) * 0x01010101u) & 0x80808080u) != 0u;
+#else
+    (void)t;
+    return false;
+#endif
+}
+
+#if SE_TBL == 2
+// Coded table: a block of codes t with hash lanes h.  Blocks holding NULL or WALL
+// (a code >= D) run the generated chain; FIXED = false where the tile holds
+// neither.  Otherwise the mirror (lane 0) permutes
+// the tuple before the lookup; the tuple's u8 entry names its outcome vector v,
+// and entry (v, class r of lanes 1..3, mirror) of the outcome table is a byte
+// selector over the block's codes and constant codes: one v_perm builds the new
+// block.
+template <bool FIXED>
+__device__ __forceinline__ u32 se_ct_eval(const u32* __restrict__ lt, u32 t, const u32* h) {
+    if (FIXED && __builtin_expect(se_has_fixed(t), 0)) {
+        u32 c0 = se_dec(t & 0xffu), c1 = se_dec((t >> 8) & 0xffu), c2 = se_dec((t >> 16) & 0xffu), c3 = se_dec(t >> 24);
+        se_block_h(c0, c1, c2, c3, h[0], h[1], h[2], h[3]);
+        return se_pack4(se_enc_ok(c0), se_enc_ok(c1), se_enc_ok(c2), se_enc_ok(c3));
+    }
+    const u32 mirror = h[0] <= SE_CT_MIRROR_T ? 1u : 0u;
+    const u32 tm = mirror ? __builtin_amdgcn_perm(t, t, 0x02030001u) : t;
+    const u32 v = reinterpret_cast<const unsigned char*>(lt)[se_tuple<SE_CT_D>(tm)];
+    const uint2 o = reinterpret_cast<const uint2*>(lt + SE_CT_T1_WORDS)[(v * SE_CT_R + se_rclass_ct(h[1], h[2], h[3])) * 2u + mirror];
+    return __builtin_amdgcn_perm(o.y, t, o.x);
+}
+#endif
+
+template <bool FIXED>
 __device__ __forceinline__ void se_blk_eval2(const u32* __restrict__ ltbl, u32 ta, u32 tb, const u32* xa,
                                              const u32* kf, int gya, int gyb, u32& ea, u32& eb) {
-#if SE_TBL
+#if SE_TBL == 1
     u32 ha[4], hb[4];
     se_hash_lanes(ha, xa, kf, gya);
     se_hash_lanes(hb, xa, kf, gyb);
-    ea = ltbl[se_tuple(ta) * SE_TBL_R + se_rclass(ha[0], ha[1], ha[2], ha[3])];
-    eb = ltbl[se_tuple(tb) * SE_TBL_R + se_rclass(hb[0], hb[1], hb[2], hb[3])];
+    ea = ltbl[se_tuple<SE_NMAT>(ta) * SE_TBL_R + se_rclass(ha[0], ha[1], ha[2], ha[3])];
+    eb = ltbl[se_tuple<SE_NMAT>(tb) * SE_TBL_R + se_rclass(hb[0], hb[1], hb[2], hb[3])];
+#elif SE_TBL == 2
+    u32 ha[4], hb[4];
+    se_hash_lanes(ha, xa, kf, gya);
+    se_hash_lanes(hb, xa, kf, gyb);
+    ea = se_ct_eval<FIXED>(ltbl, ta, ha);
+    eb = se_ct_eval<FIXED>(ltbl, tb, hb);
 #else
     (void)ltbl; (void)ta; (void)tb; (void)xa; (void)kf; (void)gya; (void)gyb;
     ea = eb = 0u;
@@ -418,8 +510,9 @@ __device__ __forceinline__ u32 se_blk_eval1(u32 t, const u32* xa, const u32* kf,
 // LDS accesses; with x offset 1 they move as bytes (misaligned 16-bit LDS
 // accesses are much slower).  In a tile that reaches past the world (EDGE),
 // cells outside the world stay WALL: the result bytes of those cells are
-// replaced by WALL before the store.
-template <int LW, int NT, bool ODD, bool EDGE>
+// replaced by WALL before the store.  FIXED: the tile may hold NULL or WALL
+// codes, which the coded table does not cover.
+template <int LW, int NT, bool ODD, bool EDGE, bool FIXED>
 __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* __restrict__ ltbl, int fx, int nbx,
                                                  int fy, int nby, int gx0, int gy0, int W, int H, u32 f) {
     constexpr int NW = NT / 64;
@@ -451,7 +544,7 @@ __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* 
             const u32 ta = se_blk_read<LW, ODD>(pa), tb = RPI == 2 ? se_blk_read<LW, ODD>(pb) : 0u;
             u32 ea, eb;
 #if SE_TBL
-            se_blk_eval2(ltbl, ta, tb, xa, kf, gya, gyb, ea, eb);
+            se_blk_eval2<FIXED>(ltbl, ta, tb, xa, kf, gya, gyb, ea, eb);
 #else
             (void)tb;
             eb = 0u;
@@ -467,18 +560,23 @@ __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* 
     }
 }
 
-// One Margolus sub-step without modifications; odd x offsets and edge tiles
-// get their own instantiations.
+// One Margolus sub-step without modifications; odd x offsets, edge tiles and
+// (coded table) interior tiles free of NULL and WALL get their own
+// instantiations.
 template <int LW, int NT>
 __device__ __forceinline__ void se_lds_substep(unsigned char* lds, const u32* __restrict__ ltbl, int fx, int nbx,
-                                               int fy, int nby, int gx0, int gy0, int W, int H, u32 f, bool edge) {
+                                               int fy, int nby, int gx0, int gy0, int W, int H, u32 f, bool edge,
+                                               bool fixed) {
     const bool odd = (fx & 1) != 0;  // block origins share the parity of the x offset
+    const bool clean = SE_TBL == 2 && !edge && !fixed;
     if (odd) {
-        if (edge) se_lds_substep_t<LW, NT, true, true>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
-        else se_lds_substep_t<LW, NT, true, false>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
+        if (edge) se_lds_substep_t<LW, NT, true, true, true>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
+        else if (clean) se_lds_substep_t<LW, NT, true, false, false>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
+        else se_lds_substep_t<LW, NT, true, false, true>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
     } else {
-        if (edge) se_lds_substep_t<LW, NT, false, true>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
-        else se_lds_substep_t<LW, NT, false, false>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
+        if (edge) se_lds_substep_t<LW, NT, false, true, true>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
+        else if (clean) se_lds_substep_t<LW, NT, false, false, false>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
+        else se_lds_substep_t<LW, NT, false, false, true>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
     }
 }
 
@@ -497,6 +595,7 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     const int gx0 = tx * TW - HB, gy0 = ty * TH - HB;
     const bool interior = gx0 >= 0 && gy0 >= 0 && gx0 + LW <= W && gy0 + LH <= H;
     u32* lds32 = reinterpret_cast<u32*>(lds);
+    bool fixed = false;  // this thread stored a NULL or WALL code (coded table)
     if (interior) {
         // whole tile inside the world: every thread issues U quad loads before
         // any of them is decoded into LDS, so a thread waits U loads at a time
@@ -512,7 +611,11 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int i = i0 + u * NT + tid;
-                if (i < TOTAL) lds32[i] = se_pack_quad(v[u]);
+                if (i < TOTAL) {
+                    const u32 q = se_pack_quad(v[u]);
+                    fixed |= se_has_fixed(q);
+                    lds32[i] = q;
+                }
             }
         }
     } else {
@@ -524,23 +627,25 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
             const int ry = i / nq;
             const int ly = ly_lo + ry, lq = lq_lo + (i - ry * nq);
             const int gx = gx0 + 4 * lq, gy = gy0 + ly;
-            u32 v0 = SE_WALLID, v1 = SE_WALLID, v2 = SE_WALLID, v3 = SE_WALLID;
+            u32 v0 = SE_LWALL, v1 = SE_LWALL, v2 = SE_LWALL, v3 = SE_LWALL;
             if (gy >= 0 && gy < H) {
                 const u32* row = in + (size_t)gy * pitch;
                 if (gx >= 0 && gx + 3 < W) {
                     const uint4 v = *reinterpret_cast<const uint4*>(row + gx);
-                    v0 = se_decode(v.x); v1 = se_decode(v.y); v2 = se_decode(v.z); v3 = se_decode(v.w);
+                    v0 = se_enc(v.x); v1 = se_enc(v.y); v2 = se_enc(v.z); v3 = se_enc(v.w);
                 } else {
-                    if (gx >= 0 && gx < W) v0 = se_decode(row[gx]);
-                    if (gx + 1 >= 0 && gx + 1 < W) v1 = se_decode(row[gx + 1]);
-                    if (gx + 2 >= 0 && gx + 2 < W) v2 = se_decode(row[gx + 2]);
-                    if (gx + 3 >= 0 && gx + 3 < W) v3 = se_decode(row[gx + 3]);
+                    if (gx >= 0 && gx < W) v0 = se_enc(row[gx]);
+                    if (gx + 1 >= 0 && gx + 1 < W) v1 = se_enc(row[gx + 1]);
+                    if (gx + 2 >= 0 && gx + 2 < W) v2 = se_enc(row[gx + 2]);
+                    if (gx + 3 >= 0 && gx + 3 < W) v3 = se_enc(row[gx + 3]);
                 }
             }
             lds32[ly * LQ + lq] = se_pack4(v0, v1, v2, v3);
         }
     }
-    __syncthreads();
+    // an interior tile runs the table without the NULL/WALL test when no cell
+    // holds either (table results never introduce them)
+    bool tile_fixed = SE_TBL == 2 ? __syncthreads_or(fixed ? 1 : 0) != 0 : (__syncthreads(), false);
     for (int s = 0; s < nsub; s++) {
         const u32 f = (u32)(frame0 + s);
         int ox, oy;
@@ -551,7 +656,7 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
         const int nb = nbx * nby;
         const bool use_mods = s == 0 && nmods > 0;
         if (!use_mods) {
-            se_lds_substep<LW, NT>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f, !interior);
+            se_lds_substep<LW, NT>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f, !interior, tile_fixed);
         } else if (nb > 0) {
             const int dby = NT / nbx, dbx = NT - dby * nbx;
             int by = tid / nbx, bx = tid - by * nbx;
@@ -570,8 +675,9 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                 } else {
                     c0 = p0[0]; c1 = p0[1]; c2 = p1[0]; c3 = p1[1];
                 }
+                c0 = se_dec(c0); c1 = se_dec(c1); c2 = se_dec(c2); c3 = se_dec(c3);
                 const int gx = gx0 + lx, gy = gy0 + ly;
-#if SE_TBL
+#if SE_TBL == 1
                 const u32 base = (c0 + SE_NMAT * (c1 + SE_NMAT * (c2 + SE_NMAT * c3))) * SE_TBL_R;
                 const u32 r = se_rclass(se_hash_lane(0, (u32)gx, (u32)gy, f), se_hash_lane(1, (u32)gx, (u32)gy, f),
                                         se_hash_lane(2, (u32)gx, (u32)gy, f), se_hash_lane(3, (u32)gx, (u32)gy, f));
@@ -587,6 +693,8 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                     if ((o = se_mod_override(mods, nmods, gx, gy + 1)) >= 0) c2 = (u32)o;
                     if ((o = se_mod_override(mods, nmods, gx + 1, gy + 1)) >= 0) c3 = (u32)o;
                 }
+                c0 = se_enc_ok(c0); c1 = se_enc_ok(c1); c2 = se_enc_ok(c2); c3 = se_enc_ok(c3);
+                fixed |= se_has_fixed(se_pack4(c0, c1, c2, c3));
                 if (interior) {
                     if (ox == 0) {
                         *reinterpret_cast<unsigned short*>(p0) = (unsigned short)(c0 | (c1 << 8));
@@ -605,7 +713,8 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                 }
             }
         }
-        __syncthreads();
+        if (SE_TBL == 2 && use_mods) tile_fixed = __syncthreads_or(tile_fixed || fixed ? 1 : 0) != 0;  // brushes may place WALL
+        else __syncthreads();
     }
     // interior store: quads from LDS, 16-byte global stores where inside the world
     constexpr int TQ = TW / 4;
@@ -624,7 +733,7 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                 const int i = i0 + u * NT + tid;
                 if (i < TOTAL) {
                     const int ly = i / TQ, lq = i - ly * TQ;
-                    const uint4 v = make_uint4(p[u] & 0xffu, (p[u] >> 8) & 0xffu, (p[u] >> 16) & 0xffu, p[u] >> 24);
+                    const uint4 v = se_unpack_quad(p[u]);
                     se_store_quad<true>(out + (size_t)(gy0 + HB + ly) * pitch + gx0 + HB + 4 * lq, v);
                 }
             }
@@ -637,7 +746,7 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
         const int ly = i / sq_hi, lq = i - ly * sq_hi;
         const int gx = gx0 + HB + 4 * lq, gy = gy0 + HB + ly;
         const u32 p = lds32[((ly + HB) * LW + HB) / 4 + lq];
-        const uint4 v = make_uint4(p & 0xffu, (p >> 8) & 0xffu, (p >> 16) & 0xffu, p >> 24);
+        const uint4 v = se_unpack_quad(p);
         u32* dst = out + (size_t)gy * pitch + gx;
         if (gx + 3 < W) {
             se_store_quad<true>(dst, v);
@@ -650,12 +759,14 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     __syncthreads();  // LDS is reused by the next item of a persistent workgroup
 }
 
-__device__ __forceinline__ const u32* se_stage_table(const u32* __restrict__ tbl, unsigned char* lds, int lds_tile,
-                                                     int nt) {
+// The table (tbl_words u32 words) into LDS after the tile.
+__device__ __forceinline__ const u32* se_stage_table(const u32* __restrict__ tbl, int tbl_words, unsigned char* lds,
+                                                     int lds_tile, int nt) {
 #if SE_TBL
     u32* ltbl = reinterpret_cast<u32*>(lds + ((lds_tile + 15) & ~15));
     // 16-byte loads, U per thread in flight before their LDS stores
-    constexpr int Q = (int)SE_TBL_ENTRIES / 4, U = 4;
+    const int Q = tbl_words / 4;
+    constexpr int U = 4;
     const uint4* src = reinterpret_cast<const uint4*>(tbl);
     uint4* dst = reinterpret_cast<uint4*>(ltbl);
     for (int i0 = 0; i0 < Q; i0 += U * nt) {
@@ -671,11 +782,11 @@ __device__ __forceinline__ const u32* se_stage_table(const u32* __restrict__ tbl
             if (i < Q) dst[i] = v[u];
         }
     }
-    for (int i = 4 * Q + (int)threadIdx.x; i < (int)SE_TBL_ENTRIES; i += nt) ltbl[i] = tbl[i];
+    for (int i = 4 * Q + (int)threadIdx.x; i < tbl_words; i += nt) ltbl[i] = tbl[i];
     __syncthreads();
     return ltbl;
 #else
-    (void)tbl; (void)lds; (void)lds_tile; (void)nt;
+    (void)tbl; (void)tbl_words; (void)lds; (void)lds_tile; (void)nt;
     return nullptr;
 #endif
 }
@@ -724,17 +835,17 @@ __device__ __forceinline__ int se_xcd_tile(int b, int n) {
 #define SE_LDS_KERNEL(TW, TH, HB, NT)                                                                         \
     extern "C" __global__ __launch_bounds__(NT) void se_step_lds_##TW##x##TH##_h##HB##_t##NT(                \
         const u32* __restrict__ in, u32* __restrict__ out, int W, int H, int pitch, int frame0, int nsub,     \
-        int tiles_x, const se_mod* __restrict__ mods, int nmods, const u32* __restrict__ tbl) {               \
+        int tiles_x, const se_mod* __restrict__ mods, int nmods, const u32* __restrict__ tbl, int tbl_words) { \
         extern __shared__ __align__(16) unsigned char se_lds[];                                               \
-        const u32* lt = se_stage_table(tbl, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT);                      \
+        const u32* lt = se_stage_table(tbl, tbl_words, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT);           \
         se_lds_body<TW, TH, HB, NT>(in, out, W, H, pitch, frame0, nsub, tiles_x,                               \
                                     se_xcd_tile((int)blockIdx.x, (int)gridDim.x), mods, nmods, lt, se_lds);     \
     }                                                                                                         \
     extern "C" __global__ __launch_bounds__(NT) void se_batch_lds_##TW##x##TH##_h##HB##_t##NT(               \
         const se_bworld* __restrict__ worlds, const int* __restrict__ prefix, int n_active, int round, int k, \
-        const u32* __restrict__ tbl) {                                                                        \
+        const u32* __restrict__ tbl, int tbl_words) {                                                         \
         extern __shared__ __align__(16) unsigned char se_lds[];                                               \
-        const u32* lt = se_stage_table(tbl, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT);                      \
+        const u32* lt = se_stage_table(tbl, tbl_words, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT);           \
         const int n_items = prefix[n_active];                                                                 \
         for (int item = (int)blockIdx.x; item < n_items; item += (int)gridDim.x) {                            \
             const int w = se_owner(prefix, n_active, item);                                                   \

@@ -49,7 +49,7 @@ char* dupstr(const std::string& s) {
 const std::vector<se::LdsVariant>& lds_variants() {
     static const std::vector<se::LdsVariant> v = {
         {112, 112, 8, 256}, {240, 112, 8, 512}, {224, 112, 8, 512}, {96, 112, 8, 256}, {224, 224, 16, 512},
-        {96, 96, 8, 256}};
+        {96, 96, 8, 256},   {112, 112, 8, 512}, {112, 112, 8, 1024}};
     return v;
 }
 
@@ -93,10 +93,11 @@ int parse_yaml(const char* yaml, size_t len, se::ParsingResult& pr, std::string*
     return SE_OK;
 }
 
-int gen_source(const se::ParsingResult& pr, std::string& src, se::HipGenInfo* info = nullptr) {
+int gen_source(const se::ParsingResult& pr, std::string& src, se::HipGenInfo* info = nullptr,
+               bool allow_ctable = true) {
     try {
-        if (info) se::hip_rules_source(pr, info);
-        src = se::hip_program_source(pr, lds_variants());
+        if (info) se::hip_rules_source(pr, info, allow_ctable);
+        src = se::hip_program_source(pr, lds_variants(), allow_ctable);
     } catch (const se::LangError& e) {
         return fail(SE_ERR_CODEGEN, e.what());
     } catch (const std::exception& e) {
@@ -110,6 +111,83 @@ uint64_t fnv1a(const std::string& s) {
     for (unsigned char c : s) h = (h ^ c) * 1099511628211ull;
     return h;
 }
+
+// The coded table's LDS form from its raw enumeration (se_build_ctable: for
+// each tuple of codes < D, class r < R of hash lanes 1..3 and mirror bit m, the
+// block result as ids in bytes).  Codes are (id - 3) mod nmat.  Layout, u32 words:
+//   T1: D^4 u8 entries, the tuple's outcome vector v (its outcome for every
+//       class), padded to 16 bytes;
+//   X:  per (v, r, m) an outcome {sel, consts}: new block = v_perm(consts, block,
+//       sel), where selector byte j < 4 takes byte j of the block's codes and
+//       4 + j constant code byte j.
+// An outcome encodes the unmirrored result relative to the tuple (a cell kept in
+// place, else the first input cell holding the same id, else a constant).  The
+// mirrored result must be the unmirrored result of the swapped tuple (c1, c0,
+// c3, c2), swapped back; its entry applies that to the unswapped block.
+// Returns false when the rule set does not fit: the mirror is not such a
+// permutation, a result holds NULL or WALL (kernels skip the NULL/WALL test in
+// tiles free of both), or more than 256 outcome vectors.
+bool compact_ctable(const std::vector<uint32_t>& raw, int D, int R, int nmat, std::vector<uint32_t>& out) {
+    const size_t tuples = (size_t)D * D * D * D;
+    const size_t t1_words = (tuples + 15) / 16 * 4;
+    auto enc = [&](uint32_t id) { return (id + (uint32_t)nmat - 3u) % (uint32_t)nmat; };
+    auto byte = [](uint32_t v, int j) { return (v >> (8 * j)) & 0xffu; };
+    static const int P[4] = {1, 0, 3, 2};  // the mirror swaps c0 <-> c1 and c2 <-> c3
+    std::map<std::vector<uint64_t>, int> vecs;
+    std::vector<uint64_t> vec(R);
+    std::vector<uint32_t> t1(t1_words, 0u), x;
+    for (size_t ti = 0; ti < tuples; ti++) {
+        uint32_t a[4], id[4];
+        size_t rest = ti;
+        for (int k = 0; k < 4; k++) {
+            a[k] = (uint32_t)(rest % D);
+            rest /= D;
+            id[k] = (a[k] + 3u) % (uint32_t)nmat;
+        }
+        const size_t tp = a[1] + (size_t)D * (a[0] + (size_t)D * (a[3] + (size_t)D * a[2]));
+        for (int r = 0; r < R; r++) {
+            const uint32_t o = raw[(ti * R + r) * 2], om = raw[(ti * R + r) * 2 + 1];
+            const uint32_t op = raw[(tp * R + r) * 2];
+            for (int j = 0; j < 4; j++) {
+                if (byte(om, j) != byte(op, P[j])) return false;
+                if (byte(o, j) == 1u || byte(o, j) == 2u) return false;  // results stay free of NULL and WALL
+            }
+            uint32_t sel = 0, cst = 0;
+            for (int j = 0; j < 4; j++) {
+                const uint32_t v = byte(o, j);
+                int src = id[j] == v ? j : -1;
+                for (int i = 0; i < 4 && src < 0; i++)
+                    if (id[i] == v) src = i;
+                if (src < 0) {
+                    src = 4 + j;
+                    cst |= enc(v) << (8 * j);
+                }
+                sel |= (uint32_t)src << (8 * j);
+            }
+            vec[r] = ((uint64_t)sel << 32) | cst;
+        }
+        auto it = vecs.find(vec);
+        if (it == vecs.end()) {
+            if (vecs.size() == 256) return false;
+            it = vecs.emplace(vec, (int)vecs.size()).first;
+            for (int r = 0; r < R; r++) {
+                const uint32_t sel = (uint32_t)(vec[r] >> 32), cst = (uint32_t)vec[r];
+                uint32_t selm = 0;
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t sv = byte(sel, P[j]);
+                    selm |= (sv < 4 ? (uint32_t)P[sv] : sv) << (8 * j);
+                }
+                x.insert(x.end(), {sel, cst, selm, cst});  // (v, r, m = 0) then (v, r, m = 1)
+            }
+        }
+        t1[ti / 4] |= (uint32_t)it->second << (8 * (ti & 3));
+    }
+    out = t1;
+    out.insert(out.end(), x.begin(), x.end());
+    return true;
+}
+
+constexpr int kCtableUnfit = 1;  // load_module: the rule set does not fit the coded table
 
 }  // namespace
 
@@ -156,6 +234,26 @@ int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
         int rc = launch(build, (unsigned)((tuples + 255) / 256), ctx->stream, m->table);
         if (rc) return rc;
         HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
+    if (info.ctable) {
+        const size_t D = (size_t)info.ct_D, R = (size_t)info.ct_R, tuples = D * D * D * D;
+        std::vector<uint32_t> raw(tuples * R * 2), words;
+        uint32_t* d_raw = nullptr;
+        HIPCHECK(hipMalloc(&d_raw, raw.size() * 4));
+        hipFunction_t build;
+        int rc = SE_OK;
+        if (hipModuleGetFunction(&build, m->mod, "se_build_ctable") != hipSuccess)
+            rc = fail(SE_ERR_JIT, "se_build_ctable missing");
+        if (!rc) rc = launch(build, (unsigned)((tuples + 255) / 256), ctx->stream, d_raw);
+        if (!rc && hipMemcpyDtoHAsync(raw.data(), d_raw, raw.size() * 4, ctx->stream) != hipSuccess)
+            rc = fail(SE_ERR_HIP, "coded table read-back failed");
+        if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(SE_ERR_HIP, "coded table build failed");
+        hipFree(d_raw);
+        if (rc) return rc;
+        if (!compact_ctable(raw, (int)D, (int)R, info.nmat, words)) return kCtableUnfit;
+        m->table_entries = words.size();
+        HIPCHECK(hipMalloc(&m->table, words.size() * 4));
+        HIPCHECK(hipMemcpy(m->table, words.data(), words.size() * 4, hipMemcpyHostToDevice));
     }
     for (auto& v : lds_variants()) {
         hipFunction_t f;
@@ -311,7 +409,8 @@ int lds_launch(se_world* w, const std::pair<se::LdsVariant, hipFunction_t>& v, i
     size_t vi = &v - &w->mod->lds[0];
     rc = launch_ex(v.second, (unsigned)(tiles_x * tiles_y), (unsigned)L.nt, w->mod->lds_bytes[vi], w->stream,
                    (const uint32_t*)w->buf[w->cur], w->buf[w->cur ^ 1], (int)w->W, (int)w->H, (int)w->pitch,
-                   (int)(w->frame + 1), n, tiles_x, (const void*)m.ptr, m.n, (const uint32_t*)w->mod->table);
+                   (int)(w->frame + 1), n, tiles_x, (const void*)m.ptr, m.n, (const uint32_t*)w->mod->table,
+                   (int)w->mod->table_entries);
     if (rc) return rc;
     w->frame += n;
     w->cur ^= 1;
@@ -454,6 +553,11 @@ int se_ruleset_from_yaml(se_ctx* ctx, const char* yaml, size_t len, se_ruleset**
     rc = gen_source(rs->pr, src, &info);
     if (rc) return rc;
     rc = load_module(ctx, src, info, rs->mod);
+    if (rc == kCtableUnfit) {  // the coded table does not fit this rule set: the generated chain
+        rc = gen_source(rs->pr, src, &info, false);
+        if (rc) return rc;
+        rc = load_module(ctx, src, info, rs->mod);
+    }
     if (rc) return rc;
     rs->ctx = ctx;
     rs->nmat = (int)rs->pr.materials.size();

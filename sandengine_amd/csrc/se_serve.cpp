@@ -368,7 +368,7 @@ int enqueue_round(se_serve* S, Slot& sl, std::vector<size_t>& active, bool* any)
         size_t b0 = ranges[gi].second.first, n = ranges[gi].second.second - b0;
         rc = launch_ex(mod->batch[vi], (unsigned)mod->lds_grid[vi], (unsigned)v.nt, mod->lds_bytes[vi], st,
                        (const void*)((BWorld*)sl.d_recs + b0), (const int*)((int*)sl.d_pre + pre_off[gi]), (int)n, 0,
-                       (int)(1 << 30), (const uint32_t*)mod->table);
+                       (int)(1 << 30), (const uint32_t*)mod->table, (int)mod->table_entries);
         if (rc) return rc;
     }
     HIPCHECK(hipEventRecord(sl.ev, st));

@@ -104,6 +104,32 @@ def test_random_vs_oracle(rsname, shape):
         np.testing.assert_array_equal(got, want, err_msg=f"{mode} sub={sub}")
 
 
+@pytest.mark.parametrize("rsname", ["four", "twelve", "features"])
+@pytest.mark.parametrize("wall_brush", [False, True])
+def test_interior_tiles_vs_oracle(rsname, wall_brush):
+    """A world with tiles wholly inside it and no NULL/WALL cells, so the LDS
+    kernel's interior tiles run without the NULL/WALL test; with wall_brush a
+    brush places WALL inside the world at a launch's first sub-step, after
+    which those tiles must take the test again."""
+    W, H = 520, 300
+    print(f"\n[gpu] interior {rsname} wall_brush={wall_brush}", flush=True)
+    rs = ruleset(rsname)
+    blob = O.compile_ruleset(L.parse_string(load.ruleset_text(rsname)))
+    rng = np.random.RandomState(4242 + wall_brush)
+    g0 = C.make_input(dict(ruleset=rsname, W=W, H=H, seed=int(rng.randint(0, 10 ** 6))))
+    steps = 29
+    mods_at = {}
+    if wall_brush:
+        mods_at = {8: [[260, 150, 0, 30, 2], [100, 200, 1, 12, 1], [400, 60, 0, 9, 2]], 16: [[300, 120, 1, 20, 0]]}
+    first = 6
+    want = g0
+    for s in range(steps):
+        want = O.step(blob, want, first + s, mods_at.get(s))
+    for mode, sub in [("blocked", 0), ("blocked", 8)]:
+        got, _, _ = run_protocol(rs, g0, first, steps, mods_at, mode, sub)
+        np.testing.assert_array_equal(got, want, err_msg=f"{mode} sub={sub}")
+
+
 def test_clear_frame_and_api_errors():
     rs = ruleset("four")
     w = S.World(ctx(), rs, 40, 30)
