@@ -65,6 +65,8 @@ struct se_module {
     size_t table_entries = 0;        // 0 when the rule set runs the generated chain
     std::vector<unsigned> lds_bytes; // dynamic LDS per LDS variant
     std::vector<int> lds_grid;       // resident workgroups per LDS variant at full occupancy (multiple of 8)
+    std::vector<int> lds_waves;      // resident wavefronts per CU per LDS variant
+    int default_lds = 0;             // LDS variant of se_world_step_blocked when none is set
     hipFunction_t plain = nullptr, fill = nullptr, fill_seeded = nullptr, copy = nullptr, fill_spec = nullptr;
     hipFunction_t render = nullptr;  // se_render: colour and light side-outputs of a step
     hipFunction_t plain_v[6] = {};  // se_step_plain_r{1,2,4}_nt{0,1}

@@ -46,6 +46,8 @@ char* dupstr(const std::string& s) {
     return p;
 }
 
+constexpr int kDefaultTile = 112, kDefaultHalo = 8;  // default single-world tile (see load_module)
+
 const std::vector<se::LdsVariant>& lds_variants() {
     static const std::vector<se::LdsVariant> v = {
         {112, 112, 8, 256}, {240, 112, 8, 512}, {224, 112, 8, 512}, {96, 112, 8, 256}, {224, 224, 16, 512},
@@ -272,8 +274,28 @@ int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
         m->batch.push_back(fb);
         m->lds_bytes.push_back(bytes);
         m->lds_grid.push_back(std::max(8, (std::max(per_cu, 1) * cus) & ~7));
+        m->lds_waves.push_back(std::max(per_cu, 1) * v.nt / 64);
     }
     if (m->lds.empty()) return fail(SE_ERR_JIT, "no LDS kernel variant fits the device's LDS");
+    // default single-world variant: 112 x 112 tiles, halo 8, with the smallest
+    // workgroup that keeps >= 24 wavefronts resident per CU (measured: the full
+    // table reaches that with 256 threads, the coded table's larger LDS
+    // footprint needs 512), else the one with the most resident wavefronts
+    m->default_lds = -1;
+    for (size_t i = 0; i < m->lds.size(); i++) {
+        const se::LdsVariant& v = m->lds[i].first;
+        if (v.tw != kDefaultTile || v.th != kDefaultTile || v.hb != kDefaultHalo) continue;
+        const bool enough = m->lds_waves[i] >= 24;
+        if (m->default_lds < 0) {
+            m->default_lds = (int)i;
+            continue;
+        }
+        const int d = m->default_lds;
+        const bool d_enough = m->lds_waves[d] >= 24;
+        if (enough && (!d_enough || v.nt < m->lds[d].first.nt)) m->default_lds = (int)i;
+        else if (!enough && !d_enough && m->lds_waves[i] > m->lds_waves[d]) m->default_lds = (int)i;
+    }
+    if (m->default_lds < 0) m->default_lds = 0;
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->cache[key] = {src, m};
     out = m;
@@ -366,18 +388,10 @@ using se_rt::plain_step;
 // the world's tuned choice if set, else the default tile (fastest measured on a
 // 16384^2 world) when it allows `sub`, else the variant with the smallest halo
 // >= sub.
-constexpr se::LdsVariant kDefaultLds = {112, 112, 8, 256};
 
 }  // namespace
 
-int se_rt::default_lds_index(const se_module* m) {
-    for (size_t i = 0; i < m->lds.size(); i++) {
-        const se::LdsVariant& v = m->lds[i].first;
-        if (v.tw == kDefaultLds.tw && v.th == kDefaultLds.th && v.hb == kDefaultLds.hb && v.nt == kDefaultLds.nt)
-            return (int)i;
-    }
-    return 0;
-}
+int se_rt::default_lds_index(const se_module* m) { return m->default_lds; }
 
 namespace {
 
@@ -387,10 +401,10 @@ const std::pair<se::LdsVariant, hipFunction_t>* pick_lds(se_world* w, uint32_t s
         const auto& p = L[w->lds_variant];
         return (sub == 0 || (uint32_t)p.first.hb >= sub) ? &p : nullptr;
     }
-    for (auto& p : L)
-        if (p.first.tw == kDefaultLds.tw && p.first.th == kDefaultLds.th && p.first.hb == kDefaultLds.hb &&
-            (sub == 0 || sub <= (uint32_t)kDefaultLds.hb))
-            return &p;
+    {
+        const auto& p = L[w->mod->default_lds];
+        if (sub == 0 || sub <= (uint32_t)p.first.hb) return &p;
+    }
     const std::pair<se::LdsVariant, hipFunction_t>* best = nullptr;
     for (auto& p : L) {
         if (sub != 0 && (uint32_t)p.first.hb < sub) continue;
