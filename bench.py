@@ -175,6 +175,70 @@ def run_batch(args):
         sys.exit(1)
 
 
+RENDER_BYTES_PER_CELL = 56  # se_render: ids now and before the step, light in; light and colour out (RGBA f32)
+
+
+def run_render(args):
+    """Grid step plus the render side-outputs of every step (colour noise and
+    light propagation) on the 16384^2 four-material world: the plain step kernel
+    then se_render, per step.  se_render dominates; its roofline uses 56
+    algorithmic bytes per cell and its launch time is measured alone with HIP
+    events.  The grid must equal a world stepped without render outputs."""
+    import sandengine_amd as S
+    ctx = S.Context(0)
+    side, rsname = WORKLOADS["single16k"][:2]
+    rs = S.RuleSet(ctx, S.bundled_ruleset(rsname))
+    cells = side * side
+    wc = make_world(S, ctx, rs, "single16k")  # HBM reference: a streaming copy of the grid
+    wc.copy_timed(4)
+    copy_ms = wc.copy_timed(args.copy_reps) / args.copy_reps
+    wc.close()
+    copy_gbs = 8.0 * cells / (copy_ms / 1e3) / 1e9
+    w = make_world(S, ctx, rs, "single16k")
+    w.set_render(True)
+    wall, gpu_ms = time_path(S, w, "plain", args.steps, args.warmup, 0)
+    reps = max(4, args.copy_reps)
+    w.render_timed(2)
+    render_ms = w.render_timed(reps) / reps
+    step_ms = gpu_ms / args.steps
+    p50, p99 = latencies(w, "plain", 0, args.latency_samples)
+    log(f"step+render: {cells * args.steps / wall / 1e9:.1f} Gcell-upd/s wall, {step_ms:.3f} ms/step by events; "
+        f"se_render alone {render_ms:.3f} ms/launch -> {RENDER_BYTES_PER_CELL * cells / (render_ms / 1e3) / 1e9:.0f} "
+        f"GB/s algorithmic")
+    identical = None
+    if not args.no_check:
+        wp = make_world(S, ctx, rs, "single16k")
+        wp.step(w.frame - wp.frame)
+        identical = bool(np.array_equal(wp.download(), w.download()))
+        wp.close()
+        log(f"grid with render outputs == grid without: {identical}")
+    w.close()
+    kernel = "se_render"
+    prof = load_profile(kernel, args.workload)
+    achieved = RENDER_BYTES_PER_CELL * cells / (render_ms / 1e3) / 1e9
+    cpu = None if args.no_cpu_baseline else cpu_baseline("single16k")
+    out = {
+        "metric": "Gcell-updates/s & HBM-roofline %; p50/p99 step latency, batched vs plain",
+        "value": cells * args.steps / wall / 1e9, "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32 grid, f32 colour and light",
+        "data": "synthetic (seeded SplitMix64 fill generated on the device)",
+        "config": {"workload": DESCR["single16k"] + "; colour and light side-outputs of every step",
+                   "path": "plain step + se_render per step", "rules": rsname, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": prof.get("bytes_per_launch"),
+                     "algorithmic_bytes_per_launch": RENDER_BYTES_PER_CELL * cells, "avg_launch_ms": render_ms},
+        "cpu_baseline": cpu,
+        "copy_reference": {"gbs": copy_gbs, "ms": copy_ms, "frac_of_peak": copy_gbs / HBM_PEAK_GBS},
+        "paths": {"step_and_render": {"gpu_ms_per_step": step_ms, "render_ms_per_launch": render_ms,
+                                      "step_latency_ms": {"p50": p50, "p99": p99}}},
+        "parity": {"grid_identical_without_render": identical},
+    }
+    print(json.dumps(out))
+    if identical is False:
+        sys.exit(1)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -248,15 +312,15 @@ print(json.dumps(dict(value=side * side * steps / dt / 1e9, unit='Gcell-updates/
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def load_profile(kernel_prefix):
-    """Per-launch figures rocprofv3 measured for a kernel (profiles/traffic.json):
-    HBM bytes (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and
-    VALU / LDS wavefront instructions."""
+def load_profile(kernel_prefix, workload):
+    """Per-launch figures rocprofv3 measured for a kernel on a workload
+    (profiles/traffic.json): HBM bytes (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration) and VALU / LDS wavefront instructions."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
         return {}
     d = json.load(open(p))
-    for k, v in d.get("kernels", {}).items():
+    for k, v in d.get("workloads", {}).get(workload, {}).get("kernels", {}).items():
         if k.startswith(kernel_prefix):
             return v
     return {}
@@ -271,7 +335,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=24)
-    ap.add_argument("--workload", default="single16k", choices=sorted(WORKLOADS) + sorted(BATCH) + ["serve"])
+    ap.add_argument("--workload", default="single16k",
+                    choices=sorted(WORKLOADS) + sorted(BATCH) + ["serve", "render16k"])
     ap.add_argument("--tune-rounds", type=int, default=2)
     ap.add_argument("--rate", type=float, default=2000.0, help="serve: requests/s (batched kernel)")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: arrival window (s)")
@@ -296,6 +361,8 @@ def main():
         return run_batch(args)
     if args.workload == "serve":
         return run_serve(args)
+    if args.workload == "render16k":
+        return run_render(args)
     import sandengine_amd as S
     ctx = S.Context(0)
     side, rsname = WORKLOADS[args.workload][:2]
@@ -351,7 +418,7 @@ def main():
     best = max(results, key=lambda m: results[m]["gcups"])
     R = results[best]
     kernel = "se_step_plain_r1_nt1" if best == "plain" else lds_kernel
-    prof = load_profile(kernel)
+    prof = load_profile(kernel, args.workload)
     traffic = prof.get("bytes_per_launch")
     valu = None
     if prof.get("valu_wave_instr_per_launch"):

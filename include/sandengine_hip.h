@@ -140,6 +140,12 @@ int se_variant_info(const se_world* w, int32_t kind, int32_t index, int32_t* a, 
 int se_world_set_render(se_world* w, int enable);
 int se_world_render_download(se_world* w, float* color, float* light);
 int se_world_render_ptrs(se_world* w, void** color, void** light, uint32_t* pitch);
+/* Timing helper, no reference counterpart (the reference renders inside every
+ * step): runs the render kernel n more times on the last step's two grids,
+ * between HIP events on the world's stream, and returns the elapsed GPU time.
+ * The grid is unchanged; the light advances n propagation steps on it and the
+ * colour is rewritten.  Requires render outputs enabled and one step taken. */
+int se_world_render_timed(se_world* w, uint32_t n, float* gpu_ms);
 /* World save/load (the reference lists "World loading/ unloading -- using
  * chunks, save all relevant textures" as a to-do, README.md:85-86): the grid,
  * the frame and the queued modifications, in a binary file written and read in

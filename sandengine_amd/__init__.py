@@ -103,6 +103,7 @@ def lib():
         L.se_world_set_render.argtypes = [c_p, ctypes.c_int]
         L.se_world_render_download.argtypes = [c_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         L.se_world_render_ptrs.argtypes = [c_p, PP, PP, ctypes.POINTER(c_u32)]
+        L.se_world_render_timed.argtypes = [c_p, c_u32, ctypes.POINTER(ctypes.c_float)]
         L.se_world_load.argtypes = [c_p, c_p, ctypes.c_char_p, PP]
         L.se_world_set_frame.argtypes = [c_p, c_i32]
         L.se_world_sync.argtypes = [c_p]
@@ -374,6 +375,13 @@ class World:
         fp = ctypes.POINTER(ctypes.c_float)
         _check(lib().se_world_render_download(self._p, col.ctypes.data_as(fp), light.ctypes.data_as(fp)))
         return col, light
+
+    def render_timed(self, n: int) -> float:
+        """GPU ms of n more render launches on the last step's grids (timing
+        only: the light advances n propagation steps, the grid is unchanged)."""
+        ms = ctypes.c_float()
+        _check(lib().se_world_render_timed(self._p, n, ctypes.byref(ms)))
+        return ms.value
 
     def save(self, path: str) -> None:
         """Writes the grid, frame and queued modifications (se_world_save)."""

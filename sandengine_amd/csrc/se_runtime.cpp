@@ -771,6 +771,29 @@ int se_world_render_download(se_world* w, float* color, float* light) {
     return SE_OK;
 }
 
+int se_world_render_timed(se_world* w, uint32_t n, float* gpu_ms) {
+    if (!w || !gpu_ms) return fail(SE_ERR_ARG, "null argument");
+    if (!w->color) return fail(SE_ERR_STATE, "render outputs are not enabled for this world");
+    if (w->frame < 1) return fail(SE_ERR_STATE, "no step has been taken");
+    DeviceGuard dg(w->ctx->device);
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    hipEventRecord(a, w->stream);
+    int rc = SE_OK;
+    for (uint32_t i = 0; i < n && !rc; i++) rc = render_step(w, w->buf[w->cur ^ 1], w->buf[w->cur]);
+    hipEventRecord(b, w->stream);
+    hipError_t e = hipEventSynchronize(b);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(SE_ERR_HIP, hipGetErrorString(e));
+    *gpu_ms = ms;
+    return SE_OK;
+}
+
 int se_world_render_ptrs(se_world* w, void** color, void** light, uint32_t* pitch) {
     if (!w) return fail(SE_ERR_ARG, "null world");
     if (!w->color) return fail(SE_ERR_STATE, "render outputs are not enabled for this world");

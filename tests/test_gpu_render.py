@@ -91,3 +91,35 @@ def test_render_through_blocked_steps_and_batch_guard():
         w.close()
     for a, b in zip(out[0], out[1]):
         np.testing.assert_array_equal(a, b)
+
+
+def test_render_timed_rerenders_without_touching_the_grid():
+    """The timing helper re-runs the render kernel on the last step's grids: the
+    grid and the colour stay as they are, the light advances one propagation
+    per launch (two launches at once == one and one); it needs render outputs."""
+    r = rs("twelve_materials")
+    ids, pct = [0] + list(range(3, 15)), [40] + [5] * 12
+    ws = []
+    for _ in range(2):
+        w = S.World(_ctx, r, 90, 60)
+        w.set_render(True)
+        w.step(1)
+        w.upload(O.fill(8, 90, 60, ids, pct))
+        w.step(6)
+        ws.append(w)
+    grid0 = ws[0].download()
+    col0, light0 = ws[0].render_download()
+    assert ws[0].render_timed(2) >= 0.0
+    ws[1].render_timed(1)
+    ws[1].render_timed(1)
+    g, (c, l) = ws[0].download(), ws[0].render_download()
+    np.testing.assert_array_equal(g, grid0)
+    np.testing.assert_array_equal(c, col0)
+    assert not np.array_equal(l, light0)
+    np.testing.assert_array_equal(l, ws[1].render_download()[1])
+    for w in ws:
+        w.close()
+    w = S.World(_ctx, r, 20, 20)
+    with pytest.raises(S.SandengineError):
+        w.render_timed(1)
+    w.close()
