@@ -736,22 +736,24 @@ std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info, bool allo
         }
         o << "}\n";
     }
-    o << "__device__ __forceinline__ float se_dens(unsigned int c) {\n    const float t[" << n << "] = {";
+    // per-material tables in constant memory (a local array indexed by a
+    // runtime id would be rebuilt in scratch memory at every use)
+    o << "__constant__ float se_dens_tab[" << n << "] = {";
     for (int i = 0; i < n; i++) o << (i ? ", " : "") << flit(ctx.dens[i]);
-    o << "};\n    return t[c];\n}\n";
-    o << "__device__ __forceinline__ unsigned int se_type(unsigned int c) {\n    const unsigned int t[" << n << "] = {";
+    o << "};\n__device__ __forceinline__ float se_dens(unsigned int c) { return se_dens_tab[c]; }\n";
+    o << "__constant__ unsigned int se_type_tab[" << n << "] = {";
     for (int i = 0; i < n; i++) o << (i ? ", " : "") << ctx.mat_type[i];
-    o << "};\n    return t[c];\n}\n";
+    o << "};\n__device__ __forceinline__ unsigned int se_type(unsigned int c) { return se_type_tab[c]; }\n";
     // render side-outputs (operations.glsl:99-170): material colour and emission,
     // and isLightObstacle (operations.glsl:68-70): no RGB emission and not of type EMPTY
     for (int k = 0; k < 2; k++) {
         const auto& tab = k == 0 ? ctx.color : ctx.emission;
-        o << "__device__ __forceinline__ float4 " << (k == 0 ? "se_color" : "se_emission")
-          << "(unsigned int c) {\n    const float4 t[" << n << "] = {";
+        const char* nm = k == 0 ? "se_color" : "se_emission";
+        o << "__constant__ float4 " << nm << "_tab[" << n << "] = {";
         for (int i = 0; i < n; i++)
             o << (i ? ", " : "") << "{" << flit(tab[i][0]) << ", " << flit(tab[i][1]) << ", " << flit(tab[i][2]) << ", "
               << flit(tab[i][3]) << "}";
-        o << "};\n    return t[c];\n}\n";
+        o << "};\n__device__ __forceinline__ float4 " << nm << "(unsigned int c) { return " << nm << "_tab[c]; }\n";
     }
     {
         int empty_type = -1;

@@ -69,6 +69,7 @@ struct se_module {
     int default_lds = 0;             // LDS variant of se_world_step_blocked when none is set
     hipFunction_t plain = nullptr, fill = nullptr, fill_seeded = nullptr, copy = nullptr, fill_spec = nullptr;
     hipFunction_t render = nullptr;  // se_render: colour and light side-outputs of a step
+    hipFunction_t noise_fill = nullptr;  // se_noise_fill: the colour noise image
     hipFunction_t plain_v[6] = {};  // se_step_plain_r{1,2,4}_nt{0,1}
     std::vector<std::pair<se::LdsVariant, hipFunction_t>> lds;
     std::vector<hipFunction_t> batch;  // se_batch_lds_* parallel to `lds`
@@ -115,6 +116,7 @@ struct se_world {
     int lds_variant = -1;   // index into se_module::lds, -1 = default choice
     // render side-outputs (se_world_set_render): RGBA f32 per cell, row pitch `pitch`
     void* light[2] = {nullptr, nullptr};  // ping-pong: light[lcur] holds the last step's light
+    float* noise = nullptr;                // colour noise per cell (position only), filled once
     void* color = nullptr;
     int lcur = 0;
 };

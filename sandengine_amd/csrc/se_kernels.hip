@@ -124,6 +124,13 @@ __device__ __forceinline__ void se_store_quad(u32* p, const uint4& q) {
     }
 }
 
+typedef float se_f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void se_store_f4(float4* p, const float4& v) {
+    se_f32x4 t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, reinterpret_cast<se_f32x4*>(p));
+}
+
 // One wavefront handles R consecutive block rows of one strip: all 2R row loads
 // are issued before any block is evaluated, to keep more bytes in flight.
 template <int R, bool NT>
@@ -917,7 +924,7 @@ extern "C" __global__ __launch_bounds__(256) void se_fill_seeded(u32* __restrict
 
 // ---------------------------------------------------------------------------
 // Render side-outputs of one step (setCell, shaders/compute/operations.glsl:99-170),
-// one thread per cell, after the step that turned `prev` into `cur`:
+// after the step that turned `prev` into `cur`:
 //   colour: the cell's material colour; unless EMPTY, rgb minus
 //           noise(pos, 3, 2.0, 0.25) * 0.25, clamped to [0, 1] (math.glsl:80-106);
 //   light:  the material's emission if its rgb is non-zero, else full light on
@@ -927,10 +934,10 @@ extern "C" __global__ __launch_bounds__(256) void se_fill_seeded(u32* __restrict
 //           order (math.glsl:154-166).
 // Float arithmetic follows the GLSL expression order with no contraction; sin
 // is this platform's sinf, so the noise matches the GL implementation's only
-// to within its sin error.
+// to within its sin error.  The noise depends on the position alone: se_noise_fill
+// evaluates it once per world into an f32 image that se_render reads.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float se_fract(float x) { return x - floorf(x); }
-
 __device__ __forceinline__ float2 se_old_hash2(float px, float py) {
     const float a = px * 127.1f + py * 311.7f, b = px * 269.5f + py * 183.3f;
     return make_float2(-1.0f + 2.0f * se_fract(sinf(a) * 43758.5453123f),
@@ -969,64 +976,144 @@ __device__ __forceinline__ float se_noise(float px, float py) {  // noise(p, 3, 
     return f;
 }
 
+// Noise image of the colour output: noise(x, y) for every cell (once per world).
+extern "C" __global__ __launch_bounds__(256) void se_noise_fill(float* __restrict__ noise, int W, int H, int pitch) {
+    const int x = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63), y = (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    if (x >= W || y >= H) return;
+    noise[(size_t)y * pitch + x] = se_noise((float)x, (float)y);
+}
+
+// One neighbour's light as setCell uses it: (rgb * pass * w, w), pass = 0 for a
+// light obstacle in the pre-step grid.  Per column, shared by the 3 cells that
+// take it as a neighbour.
+__device__ __forceinline__ float4 se_light_term(const u32* __restrict__ prev, const float4* __restrict__ light_in,
+                                                size_t q) {
+    const float pass = se_light_obstacle(se_decode(prev[q])) ? 0.0f : 1.0f;
+    float4 ld = light_in[q];
+    ld.x *= pass;
+    ld.y *= pass;
+    ld.z *= pass;
+    return make_float4(ld.x * ld.w, ld.y * ld.w, ld.z * ld.w, ld.w);
+}
+
+__device__ __forceinline__ float se_dpp_from_left(float v) {  // lane i gets lane i-1 (wave_shr:1)
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float se_dpp_from_right(float v) {  // lane i gets lane i+1 (wave_shl:1)
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+}
+
+// Light terms of one row at columns x-1, x, x+1 for the lane's column x: each
+// lane computes its own column; x-1 and x+1 come from the neighbouring lanes,
+// and lanes 0 and 63 compute the columns just outside the wavefront's strip.
+struct se_lrow {
+    float4 l, c, r;
+};
+
+__device__ __forceinline__ se_lrow se_light_row(const u32* __restrict__ prev, const float4* __restrict__ light_in,
+                                                int yy, int x, int lane, int W, int H, int pitch) {
+    se_lrow o;
+    o.l = o.c = o.r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (yy < 0 || yy >= H) return o;  // uniform across the wavefront
+    const size_t row = (size_t)yy * pitch;
+    if (x < W) o.c = se_light_term(prev, light_in, row + x);
+    float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int xe = lane == 0 ? x - 1 : x + 1;  // lanes 0 and 63: the column past the strip
+    if ((lane == 0 || lane == 63) && xe >= 0 && xe < W) e = se_light_term(prev, light_in, row + xe);
+    o.l = make_float4(se_dpp_from_left(o.c.x), se_dpp_from_left(o.c.y), se_dpp_from_left(o.c.z),
+                      se_dpp_from_left(o.c.w));
+    o.r = make_float4(se_dpp_from_right(o.c.x), se_dpp_from_right(o.c.y), se_dpp_from_right(o.c.z),
+                      se_dpp_from_right(o.c.w));
+    if (lane == 0) o.l = e;
+    if (lane == 63) o.r = e;
+    return o;
+}
+
+// One neighbour step of setCell's light loop (math.glsl:154-166 order).
+__device__ __forceinline__ void se_light_acc(float4 l, bool in, float4& avg, float4& mx, float& max_falloff, int& n) {
+    if (!in) return;
+    const float falloff = l.w == 0.0f ? max_falloff : l.w;
+    avg.x += l.x;
+    avg.y += l.y;
+    avg.z += l.z;
+    avg.w += falloff;
+    max_falloff = fmaxf(falloff, max_falloff);
+    n++;
+    mx.x = fmaxf(mx.x, l.x);
+    mx.y = fmaxf(mx.y, l.y);
+    mx.z = fmaxf(mx.z, l.z);
+    mx.w = fmaxf(mx.w, falloff);
+}
+
+// One wavefront per 64-column strip and SE_RENDER_ROWS rows, walking down the
+// rows with the light terms of rows y-1, y, y+1 in registers: every input is
+// read from HBM about once (plus 2 rows per strip), and light and colour are
+// written with 16-byte non-temporal stores.  Grid: ((W + 255) / 256,
+// ceil(H / SE_RENDER_ROWS)) workgroups of 4 wavefronts.
+#define SE_RENDER_ROWS 64
 extern "C" __global__ __launch_bounds__(256) void se_render(const u32* __restrict__ prev, const u32* __restrict__ cur,
                                                             const float4* __restrict__ light_in,
                                                             float4* __restrict__ light_out,
-                                                            float4* __restrict__ color_out, int W, int H, int pitch) {
-    const int x = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63), y = (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
-    if (x >= W || y >= H) return;
-    const size_t p = (size_t)y * pitch + x;
-    const u32 c = se_decode(cur[p]);
-    float4 col = se_color(c);
-    if (c != SE_EMPTY) {
-        const float r = se_noise((float)x, (float)y) * 0.25f;
-        col.x = fminf(fmaxf(col.x - r, 0.0f), 1.0f);
-        col.y = fminf(fmaxf(col.y - r, 0.0f), 1.0f);
-        col.z = fminf(fmaxf(col.z - r, 0.0f), 1.0f);
-    }
-    color_out[p] = col;
-    const float4 em = se_emission(c);
-    float4 light;
-    if (em.x != 0.0f || em.y != 0.0f || em.z != 0.0f) {
-        light = em;
-    } else if (y == 0) {
-        light = make_float4(1.0f, 1.0f, 1.0f, 0.999999f);
-    } else {
-        const int ndx[8] = {0, 0, -1, -1, 1, 1, 1, -1}, ndy[8] = {1, -1, 1, -1, 1, -1, 0, 0};
-        float4 avg = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mx = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float max_falloff = 0.0f;
-        int n = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int nx = x + ndx[k], ny = y + ndy[k];
-            if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
-            const size_t q = (size_t)ny * pitch + nx;
-            const float pass = se_light_obstacle(se_decode(prev[q])) ? 0.0f : 1.0f;
-            float4 ld = light_in[q];
-            ld.x *= pass;
-            ld.y *= pass;
-            ld.z *= pass;
-            const float falloff = ld.w == 0.0f ? max_falloff : ld.w;
-            const float4 l = make_float4(ld.x * ld.w, ld.y * ld.w, ld.z * ld.w, falloff);
-            avg.x += l.x;
-            avg.y += l.y;
-            avg.z += l.z;
-            avg.w += l.w;
-            max_falloff = fmaxf(falloff, max_falloff);
-            n++;
-            mx.x = fmaxf(mx.x, l.x);
-            mx.y = fmaxf(mx.y, l.y);
-            mx.z = fmaxf(mx.z, l.z);
-            mx.w = fmaxf(mx.w, l.w);
+                                                            float4* __restrict__ color_out,
+                                                            const float* __restrict__ noise, int W, int H, int pitch) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int sx = ((int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))) * 64;
+    if (sx >= W) return;
+    const int x = sx + lane;
+    const int y0 = (int)blockIdx.y * SE_RENDER_ROWS, y1 = min(H, y0 + SE_RENDER_ROWS);
+    const bool inl = x - 1 >= 0 && x - 1 < W, inr = x + 1 < W;
+    se_lrow up = se_light_row(prev, light_in, y0 - 1, x, lane, W, H, pitch);
+    se_lrow mid = se_light_row(prev, light_in, y0, x, lane, W, H, pitch);
+    for (int y = y0; y < y1; y++) {
+        const se_lrow dn = se_light_row(prev, light_in, y + 1, x, lane, W, H, pitch);
+        if (x < W) {
+            const size_t p = (size_t)y * pitch + x;
+            const u32 c = se_decode(cur[p]);
+            float4 col = se_color(c);
+            if (c != SE_EMPTY) {
+                const float r = noise[p] * 0.25f;
+                col.x = fminf(fmaxf(col.x - r, 0.0f), 1.0f);
+                col.y = fminf(fmaxf(col.y - r, 0.0f), 1.0f);
+                col.z = fminf(fmaxf(col.z - r, 0.0f), 1.0f);
+            }
+            se_store_f4(color_out + p, col);
+            const float4 em = se_emission(c);
+            float4 light;
+            if (em.x != 0.0f || em.y != 0.0f || em.z != 0.0f) {
+                light = em;
+            } else if (y == 0) {
+                light = make_float4(1.0f, 1.0f, 1.0f, 0.999999f);
+            } else {
+                const bool ind = y + 1 < H, inu = y - 1 >= 0;
+                float4 avg = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mx = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                float max_falloff = 0.0f;
+                int n = 0;
+                se_light_acc(dn.c, ind, avg, mx, max_falloff, n);         // ( 0,  1)
+                se_light_acc(up.c, inu, avg, mx, max_falloff, n);         // ( 0, -1)
+                se_light_acc(dn.l, ind && inl, avg, mx, max_falloff, n);  // (-1,  1)
+                se_light_acc(up.l, inu && inl, avg, mx, max_falloff, n);  // (-1, -1)
+                se_light_acc(dn.r, ind && inr, avg, mx, max_falloff, n);  // ( 1,  1)
+                se_light_acc(up.r, inu && inr, avg, mx, max_falloff, n);  // ( 1, -1)
+                se_light_acc(mid.r, inr, avg, mx, max_falloff, n);        // ( 1,  0)
+                se_light_acc(mid.l, inl, avg, mx, max_falloff, n);        // (-1,  0)
+                if (n == 8) {  // x / 8 == x * 0.125 exactly
+                    avg.x *= 0.125f;
+                    avg.y *= 0.125f;
+                    avg.z *= 0.125f;
+                    avg.w *= 0.125f;
+                } else if (n > 0) {
+                    const float fn = (float)n;
+                    avg.x /= fn;
+                    avg.y /= fn;
+                    avg.z /= fn;
+                    avg.w /= fn;
+                }
+                light = make_float4(avg.x * 0.5f + mx.x * 0.5f, avg.y * 0.5f + mx.y * 0.5f,
+                                    avg.z * 0.5f + mx.z * 0.5f, avg.w);
+            }
+            se_store_f4(light_out + p, light);
         }
-        if (n > 0) {
-            const float fn = (float)n;
-            avg.x /= fn;
-            avg.y /= fn;
-            avg.z /= fn;
-            avg.w /= fn;
-        }
-        light = make_float4(avg.x * 0.5f + mx.x * 0.5f, avg.y * 0.5f + mx.y * 0.5f, avg.z * 0.5f + mx.z * 0.5f, avg.w);
+        up = mid;
+        mid = dn;
     }
-    light_out[p] = light;
 }

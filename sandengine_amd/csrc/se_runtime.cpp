@@ -224,6 +224,7 @@ int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
     HIPCHECK(hipModuleGetFunction(&m->copy, m->mod, "se_copy"));
     HIPCHECK(hipModuleGetFunction(&m->fill_spec, m->mod, "se_fill_spec"));
     HIPCHECK(hipModuleGetFunction(&m->render, m->mod, "se_render"));
+    HIPCHECK(hipModuleGetFunction(&m->noise_fill, m->mod, "se_noise_fill"));
     int max_lds = 65536;
     hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
     if (info.table) {
@@ -336,11 +337,13 @@ int launch_fill(se_world* w, uint32_t* g, uint32_t v) {
 }
 
 // Colour and light of the step that turned `prev` into `cur` (render enabled).
+constexpr int kRenderRows = 64;  // rows per wavefront of se_render (SE_RENDER_ROWS)
+
 int render_step(se_world* w, const uint32_t* prev, const uint32_t* cur) {
     if (!w->color) return SE_OK;
-    dim3 grid((w->W + 63) / 64, (w->H + 3) / 4);
-    void* args[] = {(void*)&prev, (void*)&cur, &w->light[w->lcur], &w->light[w->lcur ^ 1], &w->color, (void*)&w->W,
-                    (void*)&w->H, (void*)&w->pitch};
+    dim3 grid((w->W + 255) / 256, (w->H + kRenderRows - 1) / kRenderRows);
+    void* args[] = {(void*)&prev, (void*)&cur, &w->light[w->lcur], &w->light[w->lcur ^ 1], &w->color,
+                    (void*)&w->noise, (void*)&w->W, (void*)&w->H, (void*)&w->pitch};
     HIPCHECK(hipModuleLaunchKernel(w->mod->render, grid.x, grid.y, 1, 256, 1, 1, 0, w->stream, args, nullptr));
     w->lcur ^= 1;
     return SE_OK;
@@ -645,6 +648,7 @@ void se_world_destroy(se_world* w) {
         hipFree(w->light[0]);
         hipFree(w->light[1]);
         hipFree(w->color);
+        hipFree(w->noise);
     }
     delete w;
 }
@@ -733,25 +737,32 @@ int se_world_set_render(se_world* w, int enable) {
     if (!w) return fail(SE_ERR_ARG, "null world");
     DeviceGuard dg(w->ctx->device);
     HIPCHECK(hipStreamSynchronize(w->stream));
-    if (!enable) {
+    auto release = [&]() {
         hipFree(w->light[0]);
         hipFree(w->light[1]);
         hipFree(w->color);
+        hipFree(w->noise);
         w->light[0] = w->light[1] = w->color = nullptr;
+        w->noise = nullptr;
+    };
+    if (!enable) {
+        release();
         return SE_OK;
     }
     if (w->color) return SE_OK;
     const size_t bytes = (size_t)w->pitch * w->H * 16;
     if (hipMalloc(&w->light[0], bytes) != hipSuccess || hipMalloc(&w->light[1], bytes) != hipSuccess ||
-        hipMalloc(&w->color, bytes) != hipSuccess) {
+        hipMalloc(&w->color, bytes) != hipSuccess || hipMalloc(&w->noise, bytes / 4) != hipSuccess) {
         hipGetLastError();
-        hipFree(w->light[0]);
-        hipFree(w->light[1]);
-        hipFree(w->color);
-        w->light[0] = w->light[1] = w->color = nullptr;
-        return fail(SE_ERR_OOM, "hipMalloc of " + std::to_string(3 * bytes) + " bytes for render outputs failed");
+        release();
+        return fail(SE_ERR_OOM, "hipMalloc of " + std::to_string(13 * bytes / 4) + " bytes for render outputs failed");
     }
     for (void* p : {w->light[0], w->light[1], w->color}) HIPCHECK(hipMemsetAsync(p, 0, bytes, w->stream));
+    {
+        dim3 grid((w->W + 63) / 64, (w->H + 3) / 4);
+        void* args[] = {(void*)&w->noise, (void*)&w->W, (void*)&w->H, (void*)&w->pitch};
+        HIPCHECK(hipModuleLaunchKernel(w->mod->noise_fill, grid.x, grid.y, 1, 256, 1, 1, 0, w->stream, args, nullptr));
+    }
     HIPCHECK(hipStreamSynchronize(w->stream));
     w->lcur = 0;
     return SE_OK;
