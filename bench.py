@@ -175,7 +175,9 @@ def run_batch(args):
         sys.exit(1)
 
 
-RENDER_BYTES_PER_CELL = 56  # se_render: ids now and before the step, light in; light and colour out (RGBA f32)
+# se_render per cell: id after and before the step (4 + 4), light in (16), colour
+# noise (4, precomputed per world), light and colour out (16 + 16)
+RENDER_BYTES_PER_CELL = 60
 
 
 def run_render(args):

@@ -983,13 +983,37 @@ extern "C" __global__ __launch_bounds__(256) void se_noise_fill(float* __restric
     noise[(size_t)y * pitch + x] = se_noise((float)x, (float)y);
 }
 
+// Raw inputs of one row for the light terms: the lane's column and, for lanes
+// 0 and 63, the column just outside the wavefront's strip (light, pre-step id).
+struct se_lraw {
+    float4 L, Le;
+    u32 P, Pe;
+};
+
+__device__ __forceinline__ se_lraw se_light_load(const u32* __restrict__ prev, const float4* __restrict__ light_in,
+                                                 int yy, int x, int lane, int W, int H, int pitch) {
+    se_lraw r;
+    r.L = r.Le = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    r.P = r.Pe = SE_EMPTY;
+    if (yy < 0 || yy >= H) return r;  // uniform across the wavefront
+    const size_t row = (size_t)yy * pitch;
+    if (x < W) {
+        r.L = light_in[row + x];
+        r.P = prev[row + x];
+    }
+    const int xe = lane == 0 ? x - 1 : x + 1;
+    if ((lane == 0 || lane == 63) && xe >= 0 && xe < W) {
+        r.Le = light_in[row + xe];
+        r.Pe = prev[row + xe];
+    }
+    return r;
+}
+
 // One neighbour's light as setCell uses it: (rgb * pass * w, w), pass = 0 for a
 // light obstacle in the pre-step grid.  Per column, shared by the 3 cells that
 // take it as a neighbour.
-__device__ __forceinline__ float4 se_light_term(const u32* __restrict__ prev, const float4* __restrict__ light_in,
-                                                size_t q) {
-    const float pass = se_light_obstacle(se_decode(prev[q])) ? 0.0f : 1.0f;
-    float4 ld = light_in[q];
+__device__ __forceinline__ float4 se_light_term(u32 pid, float4 ld) {
+    const float pass = se_light_obstacle(se_decode(pid)) ? 0.0f : 1.0f;
     ld.x *= pass;
     ld.y *= pass;
     ld.z *= pass;
@@ -1006,20 +1030,15 @@ __device__ __forceinline__ float se_dpp_from_right(float v) {  // lane i gets la
 // Light terms of one row at columns x-1, x, x+1 for the lane's column x: each
 // lane computes its own column; x-1 and x+1 come from the neighbouring lanes,
 // and lanes 0 and 63 compute the columns just outside the wavefront's strip.
+// A row outside the world, and columns outside it, hold zeros (never used).
 struct se_lrow {
     float4 l, c, r;
 };
 
-__device__ __forceinline__ se_lrow se_light_row(const u32* __restrict__ prev, const float4* __restrict__ light_in,
-                                                int yy, int x, int lane, int W, int H, int pitch) {
+__device__ __forceinline__ se_lrow se_light_row(const se_lraw& r, int lane) {
     se_lrow o;
-    o.l = o.c = o.r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (yy < 0 || yy >= H) return o;  // uniform across the wavefront
-    const size_t row = (size_t)yy * pitch;
-    if (x < W) o.c = se_light_term(prev, light_in, row + x);
-    float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const int xe = lane == 0 ? x - 1 : x + 1;  // lanes 0 and 63: the column past the strip
-    if ((lane == 0 || lane == 63) && xe >= 0 && xe < W) e = se_light_term(prev, light_in, row + xe);
+    o.c = se_light_term(r.P, r.L);
+    const float4 e = se_light_term(r.Pe, r.Le);
     o.l = make_float4(se_dpp_from_left(o.c.x), se_dpp_from_left(o.c.y), se_dpp_from_left(o.c.z),
                       se_dpp_from_left(o.c.w));
     o.r = make_float4(se_dpp_from_right(o.c.x), se_dpp_from_right(o.c.y), se_dpp_from_right(o.c.z),
@@ -1062,16 +1081,33 @@ extern "C" __global__ __launch_bounds__(256) void se_render(const u32* __restric
     const int x = sx + lane;
     const int y0 = (int)blockIdx.y * SE_RENDER_ROWS, y1 = min(H, y0 + SE_RENDER_ROWS);
     const bool inl = x - 1 >= 0 && x - 1 < W, inr = x + 1 < W;
-    se_lrow up = se_light_row(prev, light_in, y0 - 1, x, lane, W, H, pitch);
-    se_lrow mid = se_light_row(prev, light_in, y0, x, lane, W, H, pitch);
+    se_lrow up = se_light_row(se_light_load(prev, light_in, y0 - 1, x, lane, W, H, pitch), lane);
+    se_lrow mid = se_light_row(se_light_load(prev, light_in, y0, x, lane, W, H, pitch), lane);
+    // inputs run one row ahead of their use: row y+2's light and ids, row y+1's
+    // id and noise are in flight while row y is computed
+    se_lraw nxt = se_light_load(prev, light_in, y0 + 1, x, lane, W, H, pitch);
+    u32 cn = SE_EMPTY;
+    float nzn = 0.0f;
+    if (x < W) {
+        cn = cur[(size_t)y0 * pitch + x];
+        nzn = noise[(size_t)y0 * pitch + x];
+    }
     for (int y = y0; y < y1; y++) {
-        const se_lrow dn = se_light_row(prev, light_in, y + 1, x, lane, W, H, pitch);
+        const se_lraw nxt2 = se_light_load(prev, light_in, y + 2, x, lane, W, H, pitch);
+        const u32 cid = cn;
+        const float nz = nzn;
+        if (x < W && y + 1 < y1) {
+            cn = cur[(size_t)(y + 1) * pitch + x];
+            nzn = noise[(size_t)(y + 1) * pitch + x];
+        }
+        const se_lrow dn = se_light_row(nxt, lane);
+        nxt = nxt2;
         if (x < W) {
             const size_t p = (size_t)y * pitch + x;
-            const u32 c = se_decode(cur[p]);
+            const u32 c = se_decode(cid);
             float4 col = se_color(c);
             if (c != SE_EMPTY) {
-                const float r = noise[p] * 0.25f;
+                const float r = nz * 0.25f;
                 col.x = fminf(fmaxf(col.x - r, 0.0f), 1.0f);
                 col.y = fminf(fmaxf(col.y - r, 0.0f), 1.0f);
                 col.z = fminf(fmaxf(col.z - r, 0.0f), 1.0f);
