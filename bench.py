@@ -183,7 +183,7 @@ RENDER_BYTES_PER_CELL = 60
 def run_render(args):
     """Grid step plus the render side-outputs of every step (colour noise and
     light propagation) on the 16384^2 four-material world: the plain step kernel
-    then se_render, per step.  se_render dominates; its roofline uses 56
+    then se_render, per step.  se_render dominates; its roofline uses 60
     algorithmic bytes per cell and its launch time is measured alone with HIP
     events.  The grid must equal a world stepped without render outputs."""
     import sandengine_amd as S
