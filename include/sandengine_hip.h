@@ -95,6 +95,11 @@ void se_ruleset_destroy(se_ruleset* rs);
  * sandengine-lang/src/parser/materials.rs:52-91). */
 int se_ruleset_material_id(const se_ruleset* rs, const char* name, int32_t* id);
 int se_ruleset_material_count(const se_ruleset* rs, int32_t* n);
+/* How the LDS kernels evaluate a block for this rule set (diagnostic, no
+ * reference counterpart): 0 the generated rule chain, 1 the full transition
+ * table (N^4 x hash classes), 2 the coded table (tuples without NULL/WALL, the
+ * mirror as a byte permutation, outcome vectors).  Results are identical. */
+int se_ruleset_kernel_form(const se_ruleset* rs, int32_t* form);
 
 /* ---------------------------------------------------------------- worlds */
 /* Simulation::new (simulation.rs:128-192): frame 0, every cell EMPTY. */

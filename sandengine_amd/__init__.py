@@ -91,6 +91,7 @@ def lib():
         L.se_ruleset_destroy.argtypes = [c_p]
         L.se_ruleset_material_id.argtypes = [c_p, ctypes.c_char_p, ctypes.POINTER(c_i32)]
         L.se_ruleset_material_count.argtypes = [c_p, ctypes.POINTER(c_i32)]
+        L.se_ruleset_kernel_form.argtypes = [c_p, ctypes.POINTER(c_i32)]
         L.se_world_create.argtypes = [c_p, c_p, c_u32, c_u32, PP]
         L.se_world_destroy.argtypes = [c_p]
         L.se_world_upload.argtypes = [c_p, ctypes.POINTER(c_u32), c_sz]
@@ -296,6 +297,14 @@ class RuleSet:
         n = ctypes.c_int32()
         _check(lib().se_ruleset_material_count(self._p, ctypes.byref(n)))
         self.nmat = n.value
+
+    @property
+    def kernel_form(self) -> str:
+        """How the LDS kernels evaluate a block: "chain" (the generated rule
+        chain), "table" (full transition table) or "coded" (coded table)."""
+        f = ctypes.c_int32()
+        _check(lib().se_ruleset_kernel_form(self._p, ctypes.byref(f)))
+        return ("chain", "table", "coded")[f.value]
 
     def material_id(self, name: str) -> int:
         v = ctypes.c_int32()

@@ -63,6 +63,7 @@ struct se_module {
     hipModule_t mod = nullptr;
     uint32_t* table = nullptr;       // transition table (SE_TBL), device memory
     size_t table_entries = 0;        // 0 when the rule set runs the generated chain
+    int table_form = 0;              // 0 generated chain, 1 full table, 2 coded table
     std::vector<unsigned> lds_bytes; // dynamic LDS per LDS variant
     std::vector<int> lds_grid;       // resident workgroups per LDS variant at full occupancy (multiple of 8)
     std::vector<int> lds_waves;      // resident wavefronts per CU per LDS variant

@@ -230,6 +230,7 @@ int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
     if (info.table) {
         size_t n = (size_t)info.nmat;
         m->table_entries = n * n * n * n * (size_t)info.table_R;
+        m->table_form = 1;
         HIPCHECK(hipMalloc(&m->table, m->table_entries * 4));
         hipFunction_t build;
         HIPCHECK(hipModuleGetFunction(&build, m->mod, "se_build_table"));
@@ -255,6 +256,7 @@ int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
         if (rc) return rc;
         if (!compact_ctable(raw, (int)D, (int)R, info.nmat, words)) return kCtableUnfit;
         m->table_entries = words.size();
+        m->table_form = 2;
         HIPCHECK(hipMalloc(&m->table, words.size() * 4));
         HIPCHECK(hipMemcpy(m->table, words.data(), words.size() * 4, hipMemcpyHostToDevice));
     }
@@ -592,6 +594,12 @@ int se_ruleset_material_id(const se_ruleset* rs, const char* name, int32_t* id) 
             return SE_OK;
         }
     return fail(SE_ERR_ARG, std::string("no material named '") + name + "'");
+}
+
+int se_ruleset_kernel_form(const se_ruleset* rs, int32_t* form) {
+    if (!rs || !form) return fail(SE_ERR_ARG, "null argument");
+    *form = rs->mod->table_form;
+    return SE_OK;
 }
 
 int se_ruleset_material_count(const se_ruleset* rs, int32_t* n) {

@@ -130,6 +130,47 @@ def test_interior_tiles_vs_oracle(rsname, wall_brush):
         np.testing.assert_array_equal(got, want, err_msg=f"{mode} sub={sub}")
 
 
+def _petrify_rules():
+    """The twelve-material rule set plus a rule whose result holds WALL (acid
+    turns sand below it into WALL): too large for the full table and refused by
+    the coded table, so the LDS kernels run the generated chain."""
+    t = S.bundled_ruleset("twelve_materials")
+    t2 = t.replace("    extra_rules: [corrode]\n", "    extra_rules: [corrode, petrify]\n")
+    t2 = t2.replace("types:\n", "  petrify:\n    if: DOWN.mat == sand\n    probability: 0.05\n"
+                                 "    do: SET DOWN WALL\ntypes:\n", 1)
+    assert t2 != t
+    return t2
+
+
+def test_kernel_forms():
+    print("\n[gpu] kernel forms", flush=True)
+    assert ruleset("four").kernel_form == "table"
+    assert ruleset("twelve").kernel_form == "coded"
+    assert ruleset("features").kernel_form == "chain"  # Right rules: no mirror factoring
+    assert S.RuleSet(ctx(), _petrify_rules()).kernel_form == "chain"
+
+
+@pytest.mark.parametrize("shape", [(300, 260), (131, 77)], ids=["300x260", "131x77"])
+def test_coded_table_fallback_vs_oracle(shape):
+    """A rule set that the coded table refuses at load time still steps exactly."""
+    W, H = shape
+    print(f"\n[gpu] petrify {W}x{H}", flush=True)
+    text = _petrify_rules()
+    rs = S.RuleSet(ctx(), text)
+    blob = O.compile_ruleset(L.parse_string(text))
+    rng = np.random.RandomState(W + 17 * H)
+    g0 = O.fill(int(rng.randint(0, 10 ** 6)), W, H, [0] + list(range(3, 15)), [40] + [5] * 12)
+    steps, first = 23, 9
+    mods_at = {5: C._rand_mods(rng, W, H, 15, 3)}
+    want = g0
+    for s in range(steps):
+        want = O.step(blob, want, first + s, mods_at.get(s))
+    assert (want == 2).sum() > 0  # WALL was created
+    for mode, sub in [("plain", 0), ("blocked", 0), ("blocked", 6)]:
+        got, _, _ = run_protocol(rs, g0, first, steps, mods_at, mode, sub)
+        np.testing.assert_array_equal(got, want, err_msg=f"{mode} sub={sub}")
+
+
 def test_clear_frame_and_api_errors():
     rs = ruleset("four")
     w = S.World(ctx(), rs, 40, 30)
