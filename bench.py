@@ -132,10 +132,18 @@ def run_batch(args):
         bf.add(wf, steps)
         bs.add(ws, steps)
     t0 = time.perf_counter()
-    tuned = bf.autotune(rounds=args.tune_rounds)
+    bf.autotune(rounds=args.tune_rounds)
     tune_s = time.perf_counter() - t0
-    info = fast[0].variant_info(1, tuned[0])
-    log(f"autotune: variant {tuned[0]} {info} k={tuned[1]} persistent={tuned[2]} ({tune_s:.1f}s)")
+    tuned_by_rules = {}
+    for name, rs in rsets.items():
+        vi, k, persistent = bf.tuned(rs)
+        w0 = next(w for (_, r, _, _), w in zip(specs, fast) if r == name)
+        info = w0.variant_info(1, vi)
+        tuned_by_rules[name] = {"variant": vi, "tile": [info[0], info[1]], "halo": info[2] // 1000,
+                                "threads": info[2] % 1000, "k": k,
+                                "launch": "persistent" if persistent else "one workgroup per work item"}
+        log(f"autotune {name}: variant {vi} {info} k={k} persistent={persistent}")
+    log(f"autotune: {tune_s:.1f}s")
     t0 = time.perf_counter()
     ms_batch = bf.run(mode=1, graph=True)
     wall_batch = time.perf_counter() - t0
@@ -159,9 +167,7 @@ def run_batch(args):
         "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32", "data": "synthetic (seeded SplitMix64 fills generated on the device)",
         "config": {"workload": DESCR[args.workload], "path": "batched LDS kernel, autotuned, one hipGraph",
-                   "tuned_variant": {"tile": [info[0], info[1]], "halo": info[2] // 1000,
-                                     "threads": info[2] % 1000, "k": tuned[1],
-                                     "launch": "persistent" if tuned[2] else "one workgroup per work item"}},
+                   "tuned_variant": tuned_by_rules},
         "batched": {"gpu_ms": ms_batch, "wall_s": wall_batch, "gcups_gpu": cells / (ms_batch / 1e3) / 1e9,
                     "gcups_wall": cells / wall_batch / 1e9, "autotune_s": tune_s},
         "looped_plain": {"gpu_ms": ms_loop, "wall_s": wall_loop, "gcups_gpu": cells / (ms_loop / 1e3) / 1e9,

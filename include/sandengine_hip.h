@@ -208,6 +208,9 @@ int se_batch_run(se_batch* b, int mode, int32_t lds_variant, uint32_t k, int gra
  * for later runs.  The outputs describe the last group tuned. */
 int se_batch_autotune(se_batch* b, uint32_t rounds, int32_t* best_variant, uint32_t* best_k,
                       int32_t* best_persistent, float* best_ns_per_kcell);
+/* The tuned choice for one rule set's group (after se_batch_autotune):
+ * SE_ERR_STATE when that group has not been tuned. */
+int se_batch_tuned(const se_batch* b, const se_ruleset* rs, int32_t* variant, uint32_t* k, int32_t* persistent);
 
 /* ---------------------------------------------------------------- serving loop
  * No reference counterpart (the reference steps one interactive world per

@@ -126,6 +126,7 @@ def lib():
         L.se_fnv1a64.restype = ctypes.c_uint64
         L.se_batch_create.argtypes = [c_p, PP]
         L.se_batch_destroy.argtypes = [c_p]
+        L.se_batch_tuned.argtypes = [c_p, c_p, ctypes.POINTER(c_i32), ctypes.POINTER(c_u32), ctypes.POINTER(c_i32)]
         L.se_batch_add.argtypes = [c_p, c_p, c_u32]
         L.se_batch_clear.argtypes = [c_p]
         L.se_batch_run.argtypes = [c_p, ctypes.c_int, c_i32, c_u32, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
@@ -563,6 +564,12 @@ class Batch:
         _check(lib().se_batch_autotune(self._p, rounds, ctypes.byref(v), ctypes.byref(k), ctypes.byref(p),
                                        ctypes.byref(s)))
         return v.value, k.value, bool(p.value), s.value
+
+    def tuned(self, rules: "RuleSet") -> Tuple[int, int, bool]:
+        """(LDS variant, k, persistent launch) tuned for one rule set's group."""
+        v, k, p = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_int32()
+        _check(lib().se_batch_tuned(self._p, rules._p, ctypes.byref(v), ctypes.byref(k), ctypes.byref(p)))
+        return v.value, k.value, bool(p.value)
 
     def close(self):
         if getattr(self, "_p", None) and self._p.value:

@@ -356,6 +356,16 @@ int se_batch_run(se_batch* b, int mode, int32_t lds_variant, uint32_t k, int gra
     return run_batched(b, work, lds_variant, k, mode == 1 ? -1 : mode - 2, graph, gpu_ms);
 }
 
+int se_batch_tuned(const se_batch* b, const se_ruleset* rs, int32_t* variant, uint32_t* k, int32_t* persistent) {
+    if (!b || !rs) return fail(SE_ERR_ARG, "null argument");
+    auto it = b->tuned.find(rs->mod.get());
+    if (it == b->tuned.end()) return fail(SE_ERR_STATE, "this rule set's group has not been tuned");
+    if (variant) *variant = it->second.vi;
+    if (k) *k = it->second.k;
+    if (persistent) *persistent = it->second.persistent ? 1 : 0;
+    return SE_OK;
+}
+
 int se_batch_autotune(se_batch* b, uint32_t rounds, int32_t* best_variant, uint32_t* best_k,
                       int32_t* best_persistent, float* best_ns_per_kcell) {
     if (!b || rounds == 0) return fail(SE_ERR_ARG, "bad arguments");

@@ -84,6 +84,9 @@ def test_batch_modes_and_autotune_agree():
         b.add(w, sp[3])
     v, k, persistent, score = b.autotune(rounds=2)
     assert k >= 2 and score > 0
+    for name in sorted({sp[0] for sp in specs}):  # both rule sets have worlds that take part
+        tv, tk, tp = b.tuned(rs(name))
+        assert tv >= 0 and tk >= 2
     b.run(mode=1)
     b.run(mode=0)
     b.run(mode=2, graph=True)
