@@ -471,7 +471,9 @@ __device__ __forceinline__ u32 se_ct_eval(const u32* __restrict__ lt, u32 t, con
     const u32 mirror = h[0] <= SE_CT_MIRROR_T ? 1u : 0u;
     const u32 tm = mirror ? __builtin_amdgcn_perm(t, t, 0x02030001u) : t;
     const u32 v = reinterpret_cast<const unsigned char*>(lt)[se_tuple<SE_CT_D>(tm)];
-    const uint2 o = reinterpret_cast<const uint2*>(lt + SE_CT_T1_WORDS)[(v * SE_CT_R + se_rclass_ct(h[1], h[2], h[3])) * 2u + mirror];
+    // byte offset of entry ((v * R + r) * 2 + mirror) * 8: one 24-bit multiply-add
+    const u32 off = __umul24(v, SE_CT_R * 16u) + (se_rclass_ct(h[1], h[2], h[3]) * 16u + (mirror ? 8u : 0u));
+    const uint2 o = *reinterpret_cast<const uint2*>(reinterpret_cast<const unsigned char*>(lt + SE_CT_T1_WORDS) + off);
     return __builtin_amdgcn_perm(o.y, t, o.x);
 }
 #endif
@@ -483,8 +485,12 @@ __device__ __forceinline__ void se_blk_eval2(const u32* __restrict__ ltbl, u32 t
     u32 ha[4], hb[4];
     se_hash_lanes(ha, xa, kf, gya);
     se_hash_lanes(hb, xa, kf, gyb);
-    ea = ltbl[se_tuple<SE_NMAT>(ta) * SE_TBL_R + se_rclass(ha[0], ha[1], ha[2], ha[3])];
-    eb = ltbl[se_tuple<SE_NMAT>(tb) * SE_TBL_R + se_rclass(hb[0], hb[1], hb[2], hb[3])];
+    // byte offsets tuple * 4R + 4r (the class term as one select of constants)
+    const unsigned char* lb = reinterpret_cast<const unsigned char*>(ltbl);
+    ea = *reinterpret_cast<const u32*>(lb + se_tuple<SE_NMAT>(ta) * (SE_TBL_R * 4u) +
+                                       se_rclass(ha[0], ha[1], ha[2], ha[3]) * 4u);
+    eb = *reinterpret_cast<const u32*>(lb + se_tuple<SE_NMAT>(tb) * (SE_TBL_R * 4u) +
+                                       se_rclass(hb[0], hb[1], hb[2], hb[3]) * 4u);
 #elif SE_TBL == 2
     u32 ha[4], hb[4];
     se_hash_lanes(ha, xa, kf, gya);
