@@ -1075,7 +1075,7 @@ __device__ __forceinline__ void se_light_acc(float4 l, bool in, float4& avg, flo
 // read from HBM about once (plus 2 rows per strip), and light and colour are
 // written with 16-byte non-temporal stores.  Grid: ((W + 255) / 256,
 // ceil(H / SE_RENDER_ROWS)) workgroups of 4 wavefronts.
-#define SE_RENDER_ROWS 64
+#define SE_RENDER_ROWS 8
 extern "C" __global__ __launch_bounds__(256) void se_render(const u32* __restrict__ prev, const u32* __restrict__ cur,
                                                             const float4* __restrict__ light_in,
                                                             float4* __restrict__ light_out,

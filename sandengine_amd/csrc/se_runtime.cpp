@@ -339,7 +339,7 @@ int launch_fill(se_world* w, uint32_t* g, uint32_t v) {
 }
 
 // Colour and light of the step that turned `prev` into `cur` (render enabled).
-constexpr int kRenderRows = 64;  // rows per wavefront of se_render (SE_RENDER_ROWS)
+constexpr int kRenderRows = 8;  // rows per wavefront of se_render (SE_RENDER_ROWS)
 
 int render_step(se_world* w, const uint32_t* prev, const uint32_t* cur) {
     if (!w->color) return SE_OK;
