@@ -1072,9 +1072,11 @@ __device__ __forceinline__ void se_light_acc(float4 l, bool in, float4& avg, flo
 
 // One wavefront per 64-column strip and SE_RENDER_ROWS rows, walking down the
 // rows with the light terms of rows y-1, y, y+1 in registers: every input is
-// read from HBM about once (plus 2 rows per strip), and light and colour are
-// written with 16-byte non-temporal stores.  Grid: ((W + 255) / 256,
-// ceil(H / SE_RENDER_ROWS)) workgroups of 4 wavefronts.
+// read from HBM about once (the strip's 2 extra rows are mostly L2 hits: the
+// strip below is dispatched W/256 workgroups later, on the same XCD when W/256
+// is a multiple of 8), and light and colour are written with 16-byte
+// non-temporal stores.  Grid: ((W + 255) / 256, ceil(H / SE_RENDER_ROWS))
+// workgroups of 4 wavefronts.
 #define SE_RENDER_ROWS 8
 extern "C" __global__ __launch_bounds__(256) void se_render(const u32* __restrict__ prev, const u32* __restrict__ cur,
                                                             const float4* __restrict__ light_in,
