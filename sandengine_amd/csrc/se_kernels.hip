@@ -478,9 +478,11 @@ __device__ __forceinline__ u32 se_ct_eval(const u32* __restrict__ lt, u32 t, con
 }
 #endif
 
+// skipa / skipb: the block is left to another pass (its result is not used).
 template <bool FIXED>
 __device__ __forceinline__ void se_blk_eval2(const u32* __restrict__ ltbl, u32 ta, u32 tb, const u32* xa,
-                                             const u32* kf, int gya, int gyb, u32& ea, u32& eb) {
+                                             const u32* kf, int gya, int gyb, bool skipa, bool skipb, u32& ea,
+                                             u32& eb) {
 #if SE_TBL == 1
     u32 ha[4], hb[4];
     se_hash_lanes(ha, xa, kf, gya);
@@ -495,11 +497,14 @@ __device__ __forceinline__ void se_blk_eval2(const u32* __restrict__ ltbl, u32 t
     u32 ha[4], hb[4];
     se_hash_lanes(ha, xa, kf, gya);
     se_hash_lanes(hb, xa, kf, gyb);
-    ea = se_ct_eval<FIXED>(ltbl, ta, ha);
-    eb = se_ct_eval<FIXED>(ltbl, tb, hb);
+    ea = se_ct_eval<FIXED>(ltbl, skipa ? 0u : ta, ha);  // a skipped block reads a valid entry
+    eb = se_ct_eval<FIXED>(ltbl, skipb ? 0u : tb, hb);
 #else
     (void)ltbl; (void)ta; (void)tb; (void)xa; (void)kf; (void)gya; (void)gyb;
     ea = eb = 0u;
+#endif
+#if SE_TBL != 2
+    (void)skipa; (void)skipb;
 #endif
 }
 
@@ -525,6 +530,48 @@ __device__ __forceinline__ u32 se_blk_eval1(u32 t, const u32* xa, const u32* kf,
 // cells outside the world stay WALL: the result bytes of those cells are
 // replaced by WALL before the store.  FIXED: the tile may hold NULL or WALL
 // codes, which the coded table does not cover.
+#if SE_TBL == 2
+// Coded table, edge tiles: the blocks that touch the world border hold WALL
+// cells, which the table does not cover.  The main pass leaves them alone and
+// the workgroup evaluates them here with the generated chain, one block per
+// thread, so the chain does not hold up a whole wavefront for the one border
+// lane of every block row.  They are the first/last block row and column of
+// the sub-step's block range (blocks partition the cells, so this pass and the
+// main pass write disjoint bytes).
+template <int LW, int NT, bool ODD>
+__device__ __forceinline__ void se_lds_border(unsigned char* lds, int fx, int nbx, int fy, int nby, int gx0, int gy0,
+                                              int W, int H, u32 f) {
+    if (nbx <= 0 || nby <= 0) return;
+    const int gxl = gx0 + fx, gxr = gxl + 2 * (nbx - 1), gyt = gy0 + fy, gyb = gyt + 2 * (nby - 1);
+    const bool bl = gxl < 0, br = gxr + 1 >= W && !(bl && nbx == 1);
+    const bool bt = gyt < 0, bb = gyb + 1 >= H && !(bt && nby == 1);
+    const int nrow = (bt ? 1 : 0) + (bb ? 1 : 0), mid = nby - nrow, ncol = (bl ? 1 : 0) + (br ? 1 : 0);
+    const int n = nrow * nbx + ncol * mid;
+    for (int i = (int)threadIdx.x; i < n; i += NT) {
+        int bx, by;
+        if (i < nrow * nbx) {
+            const int r = i / nbx;
+            bx = i - r * nbx;
+            by = (r == 0 && bt) ? 0 : nby - 1;
+        } else {
+            const int j = i - nrow * nbx, c = j / mid;
+            by = (bt ? 1 : 0) + (j - c * mid);
+            bx = (c == 0 && bl) ? 0 : nbx - 1;
+        }
+        const int lx = fx + 2 * bx, ly = fy + 2 * by;
+        const int gx = gx0 + lx, gy = gy0 + ly;
+        unsigned char* p = lds + ly * LW + lx;
+        const u32 t = se_blk_read<LW, ODD>(p);
+        u32 c0 = se_dec(t & 0xffu), c1 = se_dec((t >> 8) & 0xffu), c2 = se_dec((t >> 16) & 0xffu), c3 = se_dec(t >> 24);
+        se_block(c0, c1, c2, c3, (u32)gx, (u32)gy, f);
+        const u32 colkeep =
+            ((gx >= 0 && gx < W) ? 0x00ff00ffu : 0u) | ((gx + 1 >= 0 && gx + 1 < W) ? 0xff00ff00u : 0u);
+        se_blk_write<LW, ODD>(p, se_keep_wall(se_pack4(se_enc_ok(c0), se_enc_ok(c1), se_enc_ok(c2), se_enc_ok(c3)),
+                                              colkeep, gy, H));
+    }
+}
+#endif
+
 template <int LW, int NT, bool ODD, bool EDGE, bool FIXED>
 __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* __restrict__ ltbl, int fx, int nbx,
                                                  int fy, int nby, int gx0, int gy0, int W, int H, u32 f) {
@@ -545,6 +592,10 @@ __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* 
         u32 colkeep = 0xffffffffu;
         if (EDGE)
             colkeep = ((gx >= 0 && gx < W) ? 0x00ff00ffu : 0u) | ((gx + 1 >= 0 && gx + 1 < W) ? 0xff00ff00u : 0u);
+        // coded table, edge tile: blocks touching the world border (WALL cells,
+        // which the table does not cover) are left to se_lds_border
+        constexpr bool BORDER = SE_TBL == 2 && EDGE;
+        const bool colb = BORDER && (gx < 0 || gx + 1 >= W);
         // table path: two block rows (by, by + NW) per iteration, their reads,
         // lookups and writes interleaved so the two LDS chains overlap; the
         // branchy rule chain runs one block row per iteration (measured faster)
@@ -556,21 +607,26 @@ __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* 
             unsigned char* pb = two ? pa + 2 * NW * LW : pa;
             const u32 ta = se_blk_read<LW, ODD>(pa), tb = RPI == 2 ? se_blk_read<LW, ODD>(pb) : 0u;
             u32 ea, eb;
+            const bool skipa = colb || (BORDER && (gya < 0 || gya + 1 >= H));
+            const bool skipb = colb || (BORDER && (gyb < 0 || gyb + 1 >= H));
 #if SE_TBL
-            se_blk_eval2<FIXED>(ltbl, ta, tb, xa, kf, gya, gyb, ea, eb);
+            se_blk_eval2<FIXED>(ltbl, ta, tb, xa, kf, gya, gyb, skipa, skipb, ea, eb);
 #else
             (void)tb;
             eb = 0u;
             ea = se_blk_eval1(ta, xa, kf, gya);
 #endif
-            if (EDGE) {
+            if (EDGE && !BORDER) {
                 ea = se_keep_wall(ea, colkeep, gya, H);
                 eb = se_keep_wall(eb, colkeep, gyb, H);
             }
-            se_blk_write<LW, ODD>(pa, ea);
-            if (two) se_blk_write<LW, ODD>(pb, eb);
+            if (!skipa) se_blk_write<LW, ODD>(pa, ea);
+            if (two && !skipb) se_blk_write<LW, ODD>(pb, eb);
         }
     }
+#if SE_TBL == 2
+    if (EDGE) se_lds_border<LW, NT, ODD>(lds, fx, nbx, fy, nby, gx0, gy0, W, H, f);
+#endif
 }
 
 // One Margolus sub-step without modifications; odd x offsets, edge tiles and
