@@ -857,7 +857,7 @@ std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info, bool allo
         const uint64_t D = (uint64_t)n - 2, Rc = R / 2;
         const uint64_t ct_bytes = D * D * D * D, t1_words = (ct_bytes + 15) / 16 * 4;
         const bool ct = allow_ctable && !ok && !em.rand_float_used && right.empty() && th[0].size() == 1 &&
-                        n >= 3 && D <= 127 && ct_bytes <= kCTableMaxBytes;
+                        n >= 3 && D <= 16 && ct_bytes <= kCTableMaxBytes;
         o << "#define SE_TBL " << (ok ? 1 : ct ? 2 : 0) << "\n";
         o << "#define SE_TBL_R " << R << "u\n";
         o << "#define SE_TBL_ENTRIES " << (ok ? tuples * R : 1) << "u\n";

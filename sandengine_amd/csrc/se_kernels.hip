@@ -362,15 +362,30 @@ __device__ __forceinline__ u32 se_pack4(u32 a, u32 b, u32 c, u32 d) { return a |
 
 // Four cells from HBM as LDS bytes, ids >= SE_NMAT decoded to NULL (one range
 // test for the quad; the per-cell decode runs only where one is out of range).
+// Byte-wise (b + a) mod SE_NMAT of packed bytes b < SE_NMAT, a < SE_NMAT (the
+// coded table has SE_NMAT <= 18, so no byte carries into the next).
+__device__ __forceinline__ u32 se_rot4(u32 p, u32 a) {
+    const u32 q = p + a * 0x01010101u;
+    const u32 ge = (q + (0x80u - SE_NMAT) * 0x01010101u) & 0x80808080u;  // bytes q >= SE_NMAT
+    return q - (ge >> 7) * SE_NMAT;
+}
+
 __device__ __forceinline__ u32 se_pack_quad(uint4 v) {
     const u32 mx = max(max(v.x, v.y), max(v.z, v.w));
     if (__builtin_expect(mx >= SE_NMAT, 0)) return se_pack4(se_enc(v.x), se_enc(v.y), se_enc(v.z), se_enc(v.w));
-    return se_pack4(se_enc_ok(v.x), se_enc_ok(v.y), se_enc_ok(v.z), se_enc_ok(v.w));
+#if SE_TBL == 2
+    return se_rot4(se_pack4(v.x, v.y, v.z, v.w), SE_NMAT - 3u);  // codes (id - 3) mod SE_NMAT
+#else
+    return se_pack4(v.x, v.y, v.z, v.w);
+#endif
 }
 
 // Four LDS bytes as HBM ids.
 __device__ __forceinline__ uint4 se_unpack_quad(u32 p) {
-    return make_uint4(se_dec(p & 0xffu), se_dec((p >> 8) & 0xffu), se_dec((p >> 16) & 0xffu), se_dec(p >> 24));
+#if SE_TBL == 2
+    p = se_rot4(p, 3u);  // ids (code + 3) mod SE_NMAT
+#endif
+    return make_uint4(p & 0xffu, (p >> 8) & 0xffu, (p >> 16) & 0xffu, p >> 24);
 }
 
 // Block-origin range along one axis at sub-step s: origins o (local) with parity
