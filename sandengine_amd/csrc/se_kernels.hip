@@ -342,7 +342,8 @@ extern "C" __global__ __launch_bounds__(256) void se_build_ctable(u32* __restric
 
 // ---------------------------------------------------------------------------
 // LDS temporal-blocked kernel: a workgroup of NT threads stages a
-// (TW + 2*HB) x (TH + 2*HB) tile (u8 ids) in LDS and advances it `nsub` <= HB
+// (TW + 2*HB) x (TH + 2*HB) tile (u8 ids, or u8 codes with the coded table) in
+// LDS, stages the rule set's table (if any) after it, and advances it `nsub` <= HB
 // Margolus sub-steps in place (blocks partition the cells, so in-place is
 // race-free); each sub-step evaluates only blocks whose four cells are still
 // valid, so the valid region shrinks by at most one cell per side per sub-step
