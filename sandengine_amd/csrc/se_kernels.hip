@@ -619,10 +619,11 @@ __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* 
         // which the table does not cover) are left to se_lds_border
         constexpr bool BORDER = SE_TBL == 2 && EDGE;
         const bool colb = BORDER && (gx < 0 || gx + 1 >= W);
-        // table path: two block rows (by, by + NW) per iteration, their reads,
-        // lookups and writes interleaved so the two LDS chains overlap; the
-        // branchy rule chain runs one block row per iteration (measured faster)
-        constexpr int RPI = SE_TBL ? 2 : 1;
+        // full table: two block rows (by, by + NW) per iteration, their reads,
+        // lookups and writes interleaved so the two LDS chains overlap; the coded
+        // table and the branchy rule chain run one block row per iteration (each
+        // measured faster that way)
+        constexpr int RPI = SE_TBL == 1 ? 2 : 1;
         for (int by = wv; by < nby; by += RPI * NW) {
             const bool two = RPI == 2 && by + NW < nby;  // uniform
             const int gya = gy0 + fy + 2 * by, gyb = gya + 2 * NW;
