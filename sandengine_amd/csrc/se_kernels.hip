@@ -677,7 +677,7 @@ template <int TW, int TH, int HB, int NT>
 __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __restrict__ out, int W, int H,
                                             int pitch, int frame0, int nsub, int tiles_x, int tile,
                                             const se_mod* __restrict__ mods, int nmods,
-                                            const u32* __restrict__ ltbl, unsigned char* lds) {
+                                            const u32* __restrict__ ltbl, unsigned char* lds, u32* flag, u32 seq) {
     constexpr int LW = TW + 2 * HB, LH = TH + 2 * HB, LQ = LW / 4;
     // The thread index passes through an empty asm so that per-thread offsets of
     // the tile load/store are recomputed per tile: hoisted out of a persistent
@@ -737,8 +737,12 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
         }
     }
     // an interior tile runs the table without the NULL/WALL test when no cell
-    // holds either (table results never introduce them)
-    bool tile_fixed = SE_TBL == 2 ? __syncthreads_or(fixed ? 1 : 0) != 0 : (__syncthreads(), false);
+    // holds either (table results never introduce them).  Threads that stored
+    // such a cell write this item's sequence number into the LDS flag word; an
+    // older number means clean, so the word needs no clearing between items.
+    if (SE_TBL == 2 && fixed) *flag = seq;
+    __syncthreads();
+    bool tile_fixed = SE_TBL == 2 && *flag == seq;
     for (int s = 0; s < nsub; s++) {
         const u32 f = (u32)(frame0 + s);
         int ox, oy;
@@ -806,8 +810,9 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                 }
             }
         }
-        if (SE_TBL == 2 && use_mods) tile_fixed = __syncthreads_or(tile_fixed || fixed ? 1 : 0) != 0;  // brushes may place WALL
-        else __syncthreads();
+        if (SE_TBL == 2 && use_mods && fixed) *flag = seq;  // brushes may place WALL
+        __syncthreads();
+        if (SE_TBL == 2 && use_mods) tile_fixed = *flag == seq;
     }
     // interior store: quads from LDS, 16-byte global stores where inside the world
     constexpr int TQ = TW / 4;
@@ -852,9 +857,11 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     __syncthreads();  // LDS is reused by the next item of a persistent workgroup
 }
 
-// The table (tbl_words u32 words) into LDS after the tile.
+// The table (tbl_words u32 words, a multiple of 4) into LDS after the tile; the
+// flag word after the table starts at 0.
 __device__ __forceinline__ const u32* se_stage_table(const u32* __restrict__ tbl, int tbl_words, unsigned char* lds,
-                                                     int lds_tile, int nt) {
+                                                     int lds_tile, int nt, u32* flag) {
+    if (threadIdx.x == 0) *flag = 0u;
 #if SE_TBL
     u32* ltbl = reinterpret_cast<u32*>(lds + ((lds_tile + 15) & ~15));
     // 16-byte loads, U per thread in flight before their LDS stores
@@ -880,6 +887,7 @@ __device__ __forceinline__ const u32* se_stage_table(const u32* __restrict__ tbl
     return ltbl;
 #else
     (void)tbl; (void)tbl_words; (void)lds; (void)lds_tile; (void)nt;
+    __syncthreads();
     return nullptr;
 #endif
 }
@@ -917,7 +925,8 @@ __device__ __forceinline__ int se_xcd_tile(int b, int n) {
     return (b & 7) * q + (b >> 3);
 }
 
-// dynamic LDS: tile bytes rounded to 16, then the table (SE_TBL)
+// dynamic LDS: tile bytes rounded to 16, the table (tbl_words u32, SE_TBL), a
+// 16-byte slot holding the tile's NULL/WALL flag word
 //  se_step_lds_*  : one world, one workgroup per tile (workgroups that start as
 //                   others retire keep tile loads and tile compute overlapped;
 //                   measured faster than persistent workgroups, which move in step).
@@ -930,16 +939,22 @@ __device__ __forceinline__ int se_xcd_tile(int b, int n) {
         const u32* __restrict__ in, u32* __restrict__ out, int W, int H, int pitch, int frame0, int nsub,     \
         int tiles_x, const se_mod* __restrict__ mods, int nmods, const u32* __restrict__ tbl, int tbl_words) { \
         extern __shared__ __align__(16) unsigned char se_lds[];                                               \
-        const u32* lt = se_stage_table(tbl, tbl_words, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT);           \
+        constexpr int LT = ((TW + 2 * HB) * (TH + 2 * HB) + 15) & ~15;                                         \
+        u32* fl = reinterpret_cast<u32*>(se_lds + LT + 4 * tbl_words);                                         \
+        const u32* lt = se_stage_table(tbl, tbl_words, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT, fl);       \
         se_lds_body<TW, TH, HB, NT>(in, out, W, H, pitch, frame0, nsub, tiles_x,                               \
-                                    se_xcd_tile((int)blockIdx.x, (int)gridDim.x), mods, nmods, lt, se_lds);     \
+                                    se_xcd_tile((int)blockIdx.x, (int)gridDim.x), mods, nmods, lt, se_lds, fl, \
+                                    1u);                                                                      \
     }                                                                                                         \
     extern "C" __global__ __launch_bounds__(NT) void se_batch_lds_##TW##x##TH##_h##HB##_t##NT(               \
         const se_bworld* __restrict__ worlds, const int* __restrict__ prefix, int n_active, int round, int k, \
         const u32* __restrict__ tbl, int tbl_words) {                                                         \
         extern __shared__ __align__(16) unsigned char se_lds[];                                               \
-        const u32* lt = se_stage_table(tbl, tbl_words, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT);           \
+        constexpr int LT = ((TW + 2 * HB) * (TH + 2 * HB) + 15) & ~15;                                         \
+        u32* fl = reinterpret_cast<u32*>(se_lds + LT + 4 * tbl_words);                                         \
+        const u32* lt = se_stage_table(tbl, tbl_words, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT, fl);       \
         const int n_items = prefix[n_active];                                                                 \
+        u32 seq = 0;                                                                                          \
         for (int item = (int)blockIdx.x; item < n_items; item += (int)gridDim.x) {                            \
             const int w = se_owner(prefix, n_active, item);                                                   \
             const se_bworld& B = worlds[w];                                                                   \
@@ -948,7 +963,7 @@ __device__ __forceinline__ int se_xcd_tile(int b, int n) {
             const int cur = (B.cur0 + round) & 1;                                                             \
             se_lds_body<TW, TH, HB, NT>(B.buf[cur], B.buf[cur ^ 1], B.W, B.H, B.pitch, B.frame0 + done, nsub,  \
                                         B.tiles_x, item - prefix[w], done == 0 ? B.mods : nullptr,            \
-                                        done == 0 ? B.nmods : 0, lt, se_lds);                                 \
+                                        done == 0 ? B.nmods : 0, lt, se_lds, fl, ++seq);                      \
         }                                                                                                     \
     }
 

@@ -263,8 +263,9 @@ int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
     for (auto& v : lds_variants()) {
         hipFunction_t f;
         HIPCHECK(hipModuleGetFunction(&f, m->mod, se::lds_kernel_name(v).c_str()));
+        // tile (rounded to 16 bytes), table, 16-byte slot for the tile flag word
         unsigned bytes = (unsigned)(((v.tw + 2 * v.hb) * (v.th + 2 * v.hb) + 15) & ~15) +
-                         (unsigned)(m->table_entries * 4);
+                         (unsigned)(m->table_entries * 4) + 16u;
         if ((int)bytes > max_lds) continue;  // does not fit this device's LDS
         hipFunction_t fb;
         std::string bname = se::lds_kernel_name(v);
