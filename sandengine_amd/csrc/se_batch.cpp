@@ -150,22 +150,21 @@ int make_plan(se_ctx* ctx, se_module* mod, int vi, uint32_t k, const std::vector
 int enqueue_plan(Plan& P, hipStream_t s_small, hipStream_t s_large) {
     const se::LdsVariant& v = P.mod->lds[P.vi].first;
     hipFunction_t f = P.mod->batch[P.vi];
-    const unsigned shmem = P.mod->lds_bytes[P.vi];
+    const unsigned shmem = 0;  // the LDS kernels' LDS is static
     const uint32_t* tbl = P.mod->table;
-    const int tbl_words = (int)P.mod->table_entries;
     auto grid = [&](int items) {
         return (unsigned)(P.persistent ? std::min(items, P.mod->lds_grid[P.vi]) : items);
     };
     if (!P.small.empty()) {
         int n = (int)P.small.size();
         int rc = launch_ex(f, grid(P.pre_small.back()), (unsigned)v.nt, shmem, s_small, (const void*)P.d_small.p,
-                           (const int*)P.d_pre_small.p, n, 0, (int)(1 << 30), tbl, tbl_words);
+                           (const int*)P.d_pre_small.p, n, 0, (int)(1 << 30), tbl);
         if (rc) return rc;
     }
     for (size_t r = 0; r < P.n_active.size(); r++) {
         int rc = launch_ex(f, grid(P.pre_large[P.n_active[r]]), (unsigned)v.nt, shmem, s_large,
                            (const void*)P.d_large.p, (const int*)P.d_pre_large.p, P.n_active[r], (int)r, (int)P.k,
-                           tbl, tbl_words);
+                           tbl);
         if (rc) return rc;
     }
     return SE_OK;

@@ -682,7 +682,7 @@ std::string emit_rule(const SandRule& r, const Ctx& ctx, Emitter& em) {
 
 }  // namespace
 
-std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info, bool allow_ctable) {
+std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info, bool allow_ctable, uint64_t ct_x_words) {
     Ctx ctx = build_ctx(r);
     Emitter em(ctx);
     std::string mirrored, right;
@@ -861,6 +861,9 @@ std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info, bool allo
         o << "#define SE_TBL " << (ok ? 1 : ct ? 2 : 0) << "\n";
         o << "#define SE_TBL_R " << R << "u\n";
         o << "#define SE_TBL_ENTRIES " << (ok ? tuples * R : 1) << "u\n";
+        // u32 words of the table staged into LDS (the coded table's outcome vectors
+        // are counted when the enumerated table has been compacted)
+        o << "#define SE_TBL_WORDS " << (ok ? tuples * R : ct ? t1_words + ct_x_words : 0) << "u\n";
         if (ct) {
             o << "#define SE_CT_D " << D << "u\n";
             o << "#define SE_CT_R " << Rc << "u\n";
@@ -917,8 +920,9 @@ std::string lds_kernel_name(const LdsVariant& v) {
            std::to_string(v.nt);
 }
 
-std::string hip_program_source(const ParsingResult& r, const std::vector<LdsVariant>& lds, bool allow_ctable) {
-    std::string s = hip_rules_source(r, nullptr, allow_ctable);
+std::string hip_program_source(const ParsingResult& r, const std::vector<LdsVariant>& lds, bool allow_ctable,
+                               uint64_t ct_x_words) {
+    std::string s = hip_rules_source(r, nullptr, allow_ctable, ct_x_words);
     s += "\n";
     s += kKernelTemplates;
     s += "\n";

@@ -35,10 +35,14 @@ struct HipGenInfo {
 };
 
 // Generated device code for the rule set (SE_NMAT, se_fixed, se_block, ...).
-std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info = nullptr, bool allow_ctable = true);
+// ct_x_words: u32 words of the coded table's outcome vectors, known once the
+// enumerated table has been compacted (0 for the first, enumerating build).
+std::string hip_rules_source(const ParsingResult& r, HipGenInfo* info = nullptr, bool allow_ctable = true,
+                             uint64_t ct_x_words = 0);
 
 // Full translation unit: rules + kernel templates + LDS instantiations.
-std::string hip_program_source(const ParsingResult& r, const std::vector<LdsVariant>& lds, bool allow_ctable = true);
+std::string hip_program_source(const ParsingResult& r, const std::vector<LdsVariant>& lds, bool allow_ctable = true,
+                               uint64_t ct_x_words = 0);
 
 // Name of an LDS kernel instantiation.
 std::string lds_kernel_name(const LdsVariant& v);

@@ -857,15 +857,15 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     __syncthreads();  // LDS is reused by the next item of a persistent workgroup
 }
 
-// The table (tbl_words u32 words, a multiple of 4) into LDS after the tile; the
-// flag word after the table starts at 0.
-__device__ __forceinline__ const u32* se_stage_table(const u32* __restrict__ tbl, int tbl_words, unsigned char* lds,
-                                                     int lds_tile, int nt, u32* flag) {
+// The table (SE_TBL_WORDS u32 words) into LDS after the tile; the flag word
+// after the table starts at 0.
+__device__ __forceinline__ const u32* se_stage_table(const u32* __restrict__ tbl, unsigned char* lds, int lds_tile,
+                                                     int nt, u32* flag) {
     if (threadIdx.x == 0) *flag = 0u;
 #if SE_TBL
     u32* ltbl = reinterpret_cast<u32*>(lds + ((lds_tile + 15) & ~15));
     // 16-byte loads, U per thread in flight before their LDS stores
-    const int Q = tbl_words / 4;
+    constexpr int tbl_words = (int)SE_TBL_WORDS, Q = tbl_words / 4;
     constexpr int U = 4;
     const uint4* src = reinterpret_cast<const uint4*>(tbl);
     uint4* dst = reinterpret_cast<uint4*>(ltbl);
@@ -886,7 +886,7 @@ __device__ __forceinline__ const u32* se_stage_table(const u32* __restrict__ tbl
     __syncthreads();
     return ltbl;
 #else
-    (void)tbl; (void)tbl_words; (void)lds; (void)lds_tile; (void)nt;
+    (void)tbl; (void)lds; (void)lds_tile; (void)nt;
     __syncthreads();
     return nullptr;
 #endif
@@ -925,8 +925,9 @@ __device__ __forceinline__ int se_xcd_tile(int b, int n) {
     return (b & 7) * q + (b >> 3);
 }
 
-// dynamic LDS: tile bytes rounded to 16, the table (tbl_words u32, SE_TBL), a
-// 16-byte slot holding the tile's NULL/WALL flag word
+// LDS (static, so every table offset is an immediate): tile bytes rounded to 16,
+// the table (SE_TBL_WORDS u32), a 16-byte slot holding the tile's NULL/WALL flag
+// word
 //  se_step_lds_*  : one world, one workgroup per tile (workgroups that start as
 //                   others retire keep tile loads and tile compute overlapped;
 //                   measured faster than persistent workgroups, which move in step).
@@ -937,22 +938,22 @@ __device__ __forceinline__ int se_xcd_tile(int b, int n) {
 #define SE_LDS_KERNEL(TW, TH, HB, NT)                                                                         \
     extern "C" __global__ __launch_bounds__(NT) void se_step_lds_##TW##x##TH##_h##HB##_t##NT(                \
         const u32* __restrict__ in, u32* __restrict__ out, int W, int H, int pitch, int frame0, int nsub,     \
-        int tiles_x, const se_mod* __restrict__ mods, int nmods, const u32* __restrict__ tbl, int tbl_words) { \
-        extern __shared__ __align__(16) unsigned char se_lds[];                                               \
+        int tiles_x, const se_mod* __restrict__ mods, int nmods, const u32* __restrict__ tbl) {               \
         constexpr int LT = ((TW + 2 * HB) * (TH + 2 * HB) + 15) & ~15;                                         \
-        u32* fl = reinterpret_cast<u32*>(se_lds + LT + 4 * tbl_words);                                         \
-        const u32* lt = se_stage_table(tbl, tbl_words, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT, fl);       \
+        __shared__ __align__(16) unsigned char se_lds[LT + 4 * SE_TBL_WORDS + 16];                            \
+        u32* fl = reinterpret_cast<u32*>(se_lds + LT + 4 * SE_TBL_WORDS);                                      \
+        const u32* lt = se_stage_table(tbl, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT, fl);                  \
         se_lds_body<TW, TH, HB, NT>(in, out, W, H, pitch, frame0, nsub, tiles_x,                               \
                                     se_xcd_tile((int)blockIdx.x, (int)gridDim.x), mods, nmods, lt, se_lds, fl, \
                                     1u);                                                                      \
     }                                                                                                         \
     extern "C" __global__ __launch_bounds__(NT) void se_batch_lds_##TW##x##TH##_h##HB##_t##NT(               \
         const se_bworld* __restrict__ worlds, const int* __restrict__ prefix, int n_active, int round, int k, \
-        const u32* __restrict__ tbl, int tbl_words) {                                                         \
-        extern __shared__ __align__(16) unsigned char se_lds[];                                               \
+        const u32* __restrict__ tbl) {                                                                        \
         constexpr int LT = ((TW + 2 * HB) * (TH + 2 * HB) + 15) & ~15;                                         \
-        u32* fl = reinterpret_cast<u32*>(se_lds + LT + 4 * tbl_words);                                         \
-        const u32* lt = se_stage_table(tbl, tbl_words, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT, fl);       \
+        __shared__ __align__(16) unsigned char se_lds[LT + 4 * SE_TBL_WORDS + 16];                            \
+        u32* fl = reinterpret_cast<u32*>(se_lds + LT + 4 * SE_TBL_WORDS);                                      \
+        const u32* lt = se_stage_table(tbl, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT, fl);                  \
         const int n_items = prefix[n_active];                                                                 \
         u32 seq = 0;                                                                                          \
         for (int item = (int)blockIdx.x; item < n_items; item += (int)gridDim.x) {                            \

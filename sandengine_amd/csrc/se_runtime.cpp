@@ -47,6 +47,7 @@ char* dupstr(const std::string& s) {
 }
 
 constexpr int kDefaultTile = 112, kDefaultHalo = 8;  // default single-world tile (see load_module)
+constexpr unsigned kMaxLdsBytes = 160 * 1024;        // LDS per workgroup on gfx950
 
 const std::vector<se::LdsVariant>& lds_variants() {
     static const std::vector<se::LdsVariant> v = {
@@ -57,6 +58,28 @@ const std::vector<se::LdsVariant>& lds_variants() {
 
 // hiprtc options: gfx950, IEEE f32 with no contraction, packed-f32 forms off, and
 // no merging of byte LDS accesses into misaligned wider ones (slow on LDS)
+// Static LDS of an LDS kernel variant: tile (rounded to 16 bytes), the table,
+// a 16-byte slot for the tile flag word.
+unsigned lds_bytes_of(const se::LdsVariant& v, uint64_t table_words) {
+    return (unsigned)(((v.tw + 2 * v.hb) * (v.th + 2 * v.hb) + 15) & ~15) + (unsigned)(table_words * 4) + 16u;
+}
+
+// The variants generated for a rule set whose table has table_words u32 words:
+// those whose LDS fits a workgroup.
+std::vector<se::LdsVariant> variants_for(uint64_t table_words) {
+    std::vector<se::LdsVariant> out;
+    for (auto& v : lds_variants())
+        if (lds_bytes_of(v, table_words) <= kMaxLdsBytes) out.push_back(v);
+    return out;
+}
+
+uint64_t table_words_of(const se::HipGenInfo& info, uint64_t ct_x_words) {
+    const uint64_t n = (uint64_t)info.nmat, D = (uint64_t)info.ct_D;
+    if (info.table) return n * n * n * n * (uint64_t)info.table_R;
+    if (info.ctable) return (D * D * D * D + 15) / 16 * 4 + ct_x_words;
+    return 0;
+}
+
 const char* kJitOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
                           "-Wno-parentheses-equality", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                           "-Xclang", "-target-feature", "-Xclang", "-unaligned-access-mode"};
@@ -96,10 +119,14 @@ int parse_yaml(const char* yaml, size_t len, se::ParsingResult& pr, std::string*
 }
 
 int gen_source(const se::ParsingResult& pr, std::string& src, se::HipGenInfo* info = nullptr,
-               bool allow_ctable = true) {
+               bool allow_ctable = true, uint64_t ct_x_words = 0, bool with_lds = true) {
     try {
-        if (info) se::hip_rules_source(pr, info, allow_ctable);
-        src = se::hip_program_source(pr, lds_variants(), allow_ctable);
+        se::HipGenInfo local;
+        se::HipGenInfo& gi = info ? *info : local;
+        se::hip_rules_source(pr, &gi, allow_ctable, ct_x_words);
+        src = se::hip_program_source(pr, with_lds ? variants_for(table_words_of(gi, ct_x_words))
+                                                  : std::vector<se::LdsVariant>(),
+                                     allow_ctable, ct_x_words);
     } catch (const se::LangError& e) {
         return fail(SE_ERR_CODEGEN, e.what());
     } catch (const std::exception& e) {
@@ -189,14 +216,46 @@ bool compact_ctable(const std::vector<uint32_t>& raw, int D, int R, int nmat, st
     return true;
 }
 
-constexpr int kCtableUnfit = 1;  // load_module: the rule set does not fit the coded table
+constexpr int kCtableUnfit = 1;  // build_ctable: the rule set does not fit the coded table
+
+// The coded table of a rule set: builds a module without LDS kernels, runs its
+// se_build_ctable and compacts the result on the host (compact_ctable).
+int build_ctable(se_ctx* ctx, const se::ParsingResult& pr, const se::HipGenInfo& info,
+                 std::vector<uint32_t>& words) {
+    std::string src;
+    int rc = gen_source(pr, src, nullptr, true, 0, false);
+    if (rc) return rc;
+    std::vector<char> code;
+    rc = jit_compile(src, code);
+    if (rc) return rc;
+    DeviceGuard dg(ctx->device);
+    hipModule_t mod = nullptr;
+    HIPCHECK(hipModuleLoadData(&mod, code.data()));
+    const size_t D = (size_t)info.ct_D, R = (size_t)info.ct_R, tuples = D * D * D * D;
+    std::vector<uint32_t> raw(tuples * R * 2);
+    uint32_t* d_raw = nullptr;
+    hipFunction_t build;
+    if (hipModuleGetFunction(&build, mod, "se_build_ctable") != hipSuccess)
+        rc = fail(SE_ERR_JIT, "se_build_ctable missing");
+    if (!rc && hipMalloc(&d_raw, raw.size() * 4) != hipSuccess) rc = fail(SE_ERR_OOM, "coded table buffer");
+    if (!rc) rc = launch(build, (unsigned)((tuples + 255) / 256), ctx->stream, d_raw);
+    if (!rc && hipMemcpyDtoHAsync(raw.data(), d_raw, raw.size() * 4, ctx->stream) != hipSuccess)
+        rc = fail(SE_ERR_HIP, "coded table read-back failed");
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(SE_ERR_HIP, "coded table build failed");
+    (void)hipFree(d_raw);
+    (void)hipModuleUnload(mod);
+    if (rc) return rc;
+    if (!compact_ctable(raw, (int)D, (int)R, info.nmat, words)) return kCtableUnfit;
+    return variants_for(words.size()).empty() ? kCtableUnfit : SE_OK;  // no tile fits beside it
+}
 
 }  // namespace
 
 namespace {
 
+// ctwords: the compacted coded table when the source was generated for one.
 int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
-                std::shared_ptr<se_module>& out) {
+                const std::vector<uint32_t>* ctwords, std::shared_ptr<se_module>& out) {
     uint64_t key = fnv1a(src);
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
@@ -240,39 +299,24 @@ int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
         HIPCHECK(hipStreamSynchronize(ctx->stream));
     }
     if (info.ctable) {
-        const size_t D = (size_t)info.ct_D, R = (size_t)info.ct_R, tuples = D * D * D * D;
-        std::vector<uint32_t> raw(tuples * R * 2), words;
-        uint32_t* d_raw = nullptr;
-        HIPCHECK(hipMalloc(&d_raw, raw.size() * 4));
-        hipFunction_t build;
-        int rc = SE_OK;
-        if (hipModuleGetFunction(&build, m->mod, "se_build_ctable") != hipSuccess)
-            rc = fail(SE_ERR_JIT, "se_build_ctable missing");
-        if (!rc) rc = launch(build, (unsigned)((tuples + 255) / 256), ctx->stream, d_raw);
-        if (!rc && hipMemcpyDtoHAsync(raw.data(), d_raw, raw.size() * 4, ctx->stream) != hipSuccess)
-            rc = fail(SE_ERR_HIP, "coded table read-back failed");
-        if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(SE_ERR_HIP, "coded table build failed");
-        hipFree(d_raw);
-        if (rc) return rc;
-        if (!compact_ctable(raw, (int)D, (int)R, info.nmat, words)) return kCtableUnfit;
+        if (!ctwords) return fail(SE_ERR_JIT, "internal: coded-table source without its table");
+        const std::vector<uint32_t>& words = *ctwords;
         m->table_entries = words.size();
         m->table_form = 2;
         HIPCHECK(hipMalloc(&m->table, words.size() * 4));
         HIPCHECK(hipMemcpy(m->table, words.data(), words.size() * 4, hipMemcpyHostToDevice));
     }
-    for (auto& v : lds_variants()) {
+    for (auto& v : variants_for(m->table_entries)) {  // the variants the source was generated with
         hipFunction_t f;
         HIPCHECK(hipModuleGetFunction(&f, m->mod, se::lds_kernel_name(v).c_str()));
-        // tile (rounded to 16 bytes), table, 16-byte slot for the tile flag word
-        unsigned bytes = (unsigned)(((v.tw + 2 * v.hb) * (v.th + 2 * v.hb) + 15) & ~15) +
-                         (unsigned)(m->table_entries * 4) + 16u;
+        const unsigned bytes = lds_bytes_of(v, m->table_entries);
         if ((int)bytes > max_lds) continue;  // does not fit this device's LDS
         hipFunction_t fb;
         std::string bname = se::lds_kernel_name(v);
         bname.replace(0, std::strlen("se_step_lds_"), "se_batch_lds_");
         HIPCHECK(hipModuleGetFunction(&fb, m->mod, bname.c_str()));
         int per_cu = 0, cus = 0;
-        HIPCHECK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, v.nt, bytes));
+        HIPCHECK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, v.nt, 0));
         HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
         m->lds.push_back({v, f});
         m->batch.push_back(fb);
@@ -426,11 +470,9 @@ int lds_launch(se_world* w, const std::pair<se::LdsVariant, hipFunction_t>& v, i
     if (rc) return rc;
     const se::LdsVariant& L = v.first;
     int tiles_x = (int)((w->W + L.tw - 1) / L.tw), tiles_y = (int)((w->H + L.th - 1) / L.th);
-    size_t vi = &v - &w->mod->lds[0];
-    rc = launch_ex(v.second, (unsigned)(tiles_x * tiles_y), (unsigned)L.nt, w->mod->lds_bytes[vi], w->stream,
+    rc = launch_ex(v.second, (unsigned)(tiles_x * tiles_y), (unsigned)L.nt, 0u, w->stream,
                    (const uint32_t*)w->buf[w->cur], w->buf[w->cur ^ 1], (int)w->W, (int)w->H, (int)w->pitch,
-                   (int)(w->frame + 1), n, tiles_x, (const void*)m.ptr, m.n, (const uint32_t*)w->mod->table,
-                   (int)w->mod->table_entries);
+                   (int)(w->frame + 1), n, tiles_x, (const void*)m.ptr, m.n, (const uint32_t*)w->mod->table);
     if (rc) return rc;
     w->frame += n;
     w->cur ^= 1;
@@ -572,11 +614,21 @@ int se_ruleset_from_yaml(se_ctx* ctx, const char* yaml, size_t len, se_ruleset**
     se::HipGenInfo info;
     rc = gen_source(rs->pr, src, &info);
     if (rc) return rc;
-    rc = load_module(ctx, src, info, rs->mod);
-    if (rc == kCtableUnfit) {  // the coded table does not fit this rule set: the generated chain
-        rc = gen_source(rs->pr, src, &info, false);
-        if (rc) return rc;
-        rc = load_module(ctx, src, info, rs->mod);
+    if (info.ctable) {
+        // the coded table is enumerated by a first build; the LDS kernels are then
+        // generated with its size (static LDS), or without it when it does not fit
+        std::vector<uint32_t> words;
+        rc = build_ctable(ctx, rs->pr, info, words);
+        if (rc == kCtableUnfit) {
+            rc = gen_source(rs->pr, src, &info, false);
+            if (!rc) rc = load_module(ctx, src, info, nullptr, rs->mod);
+        } else if (!rc) {
+            const uint64_t t1_words = ((uint64_t)info.ct_D * info.ct_D * info.ct_D * info.ct_D + 15) / 16 * 4;
+            rc = gen_source(rs->pr, src, &info, true, words.size() - t1_words);
+            if (!rc) rc = load_module(ctx, src, info, &words, rs->mod);
+        }
+    } else {
+        rc = load_module(ctx, src, info, nullptr, rs->mod);
     }
     if (rc) return rc;
     rs->ctx = ctx;

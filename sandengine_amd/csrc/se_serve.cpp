@@ -366,9 +366,9 @@ int enqueue_round(se_serve* S, Slot& sl, std::vector<size_t>& active, bool* any)
         int vi = lds_index(mod);
         const se::LdsVariant& v = mod->lds[vi].first;
         size_t b0 = ranges[gi].second.first, n = ranges[gi].second.second - b0;
-        rc = launch_ex(mod->batch[vi], (unsigned)mod->lds_grid[vi], (unsigned)v.nt, mod->lds_bytes[vi], st,
+        rc = launch_ex(mod->batch[vi], (unsigned)mod->lds_grid[vi], (unsigned)v.nt, 0u, st,
                        (const void*)((BWorld*)sl.d_recs + b0), (const int*)((int*)sl.d_pre + pre_off[gi]), (int)n, 0,
-                       (int)(1 << 30), (const uint32_t*)mod->table, (int)mod->table_entries);
+                       (int)(1 << 30), (const uint32_t*)mod->table);
         if (rc) return rc;
     }
     HIPCHECK(hipEventRecord(sl.ev, st));
