@@ -689,7 +689,34 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     const bool interior = gx0 >= 0 && gy0 >= 0 && gx0 + LW <= W && gy0 + LH <= H;
     u32* lds32 = reinterpret_cast<u32*>(lds);
     bool fixed = false;  // this thread stored a NULL or WALL code (coded table)
-    if (interior) {
+    // whole tile inside the world, workgroup a whole number of tile rows wide:
+    // each thread keeps one quad column and steps down the rows by pointer
+    // increments (no per-quad index arithmetic), U loads in flight before any is
+    // decoded into LDS
+    constexpr int DROW = NT % LQ == 0 ? NT / LQ : 1, KMAX = LH / DROW;
+    constexpr bool ROWS = NT % LQ == 0 && LH % DROW == 0 && KMAX % 8 == 0;
+    if (ROWS && interior) {
+        constexpr int U = 8;
+        const int lq = tid % LQ, ly0 = tid / LQ;
+        const u32* src = in + (size_t)(gy0 + ly0) * pitch + gx0 + 4 * lq;
+        const size_t sstep = (size_t)DROW * pitch;
+        u32* dst = lds32 + ly0 * LQ + lq;
+#pragma unroll
+        for (int k0 = 0; k0 < KMAX; k0 += U) {
+            uint4 v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                v[u] = *reinterpret_cast<const uint4*>(src);
+                src += sstep;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const u32 q = se_pack_quad(v[u]);
+                fixed |= se_has_fixed(q);
+                dst[(k0 + u) * DROW * LQ] = q;
+            }
+        }
+    } else if (interior) {
         // whole tile inside the world: every thread issues U quad loads before
         // any of them is decoded into LDS, so a thread waits U loads at a time
         constexpr int TOTAL = LQ * LH, U = 8;
@@ -816,6 +843,35 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     }
     // interior store: quads from LDS, 16-byte global stores where inside the world
     constexpr int TQ = TW / 4;
+    // rows mapping: threads take quad column tid % QW (QW a power of two >= TQ;
+    // columns >= TQ idle) and step down the rows by pointer increments
+    constexpr int QW = TQ <= 16 ? 16 : TQ <= 32 ? 32 : TQ <= 64 ? 64 : 128;
+    constexpr int DR = NT % QW == 0 ? NT / QW : 1, KS = TH / DR;
+    constexpr bool SROWS = NT % QW == 0 && TH % DR == 0;
+    if (SROWS && gx0 + HB + TW <= W && gy0 + HB + TH <= H) {
+        const int lq = tid % QW, ly0 = tid / QW;
+        if (lq < TQ) {
+            const u32* s = lds32 + (ly0 + HB) * LQ + HB / 4 + lq;
+            u32* d = out + (size_t)(gy0 + HB + ly0) * pitch + gx0 + HB + 4 * lq;
+            const size_t dstep = (size_t)DR * pitch;
+            constexpr int U = 4;
+#pragma unroll
+            for (int k0 = 0; k0 < KS; k0 += U) {
+                u32 p[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) p[u] = k0 + u < KS ? s[(k0 + u) * DR * LQ] : 0u;
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (k0 + u < KS) {
+                        se_store_quad<true>(d, se_unpack_quad(p[u]));
+                        d += dstep;
+                    }
+                }
+            }
+        }
+        __syncthreads();  // LDS is reused by the next item of a persistent workgroup
+        return;
+    }
     if (gx0 + HB + TW <= W && gy0 + HB + TH <= H) {
         constexpr int TOTAL = TQ * TH, U = 4;
         for (int i0 = 0; i0 < TOTAL; i0 += U * NT) {
