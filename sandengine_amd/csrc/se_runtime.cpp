@@ -52,7 +52,7 @@ constexpr unsigned kMaxLdsBytes = 160 * 1024;        // LDS per workgroup on gfx
 const std::vector<se::LdsVariant>& lds_variants() {
     static const std::vector<se::LdsVariant> v = {
         {112, 112, 8, 256}, {240, 112, 8, 512}, {224, 112, 8, 512}, {96, 112, 8, 256}, {224, 224, 16, 512},
-        {96, 96, 8, 256},   {112, 112, 8, 512}, {112, 112, 8, 1024}};
+        {96, 96, 8, 256},   {112, 112, 8, 512}};
     return v;
 }
 
