@@ -738,6 +738,49 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                 }
             }
         }
+    } else if (ROWS) {
+        // edge tile, rows mapping: the quads that touch the world or its one-cell
+        // WALL border.  U loads are in flight at a time from addresses clamped into
+        // the world's rows (always valid; rows are at least 4 cells of pitch); a
+        // quad wholly inside the world takes the loaded value, a quad across the
+        // world's side is read cell by cell, a row outside the world is WALL.
+        constexpr int U = 8;
+        const int ly_lo = max(0, -1 - gy0), ly_hi = min(LH, H + 1 - gy0);
+        const int lq_lo = max(0, (-1 - gx0) >> 2), lq_hi = min(LQ, (W + 1 - gx0 + 3) >> 2);
+        const int lq = tid % LQ, ly0 = tid / LQ;
+        const int gx = gx0 + 4 * lq;
+        const bool qin = lq >= lq_lo && lq < lq_hi;
+        const bool xfull = gx >= 0 && gx + 3 < W;
+        const int gxc = xfull ? gx : 0;
+#pragma unroll
+        for (int k0 = 0; k0 < KMAX; k0 += U) {
+            uint4 v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int gy = gy0 + ly0 + (k0 + u) * DROW;
+                const int gyc = min(max(gy, 0), H - 1);
+                v[u] = *reinterpret_cast<const uint4*>(in + (size_t)gyc * pitch + gxc);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int ly = ly0 + (k0 + u) * DROW, gy = gy0 + ly;
+                if (qin && ly >= ly_lo && ly < ly_hi) {
+                    u32 q = SE_LWALL * 0x01010101u;
+                    if (gy >= 0 && gy < H) {
+                        if (xfull) {
+                            q = se_pack_quad(v[u]);
+                        } else {
+                            const u32* row = in + (size_t)gy * pitch;
+                            u32 c[4];
+#pragma unroll
+                            for (int j = 0; j < 4; j++) c[j] = gx + j >= 0 && gx + j < W ? se_enc(row[gx + j]) : SE_LWALL;
+                            q = se_pack4(c[0], c[1], c[2], c[3]);
+                        }
+                    }
+                    lds32[ly * LQ + lq] = q;
+                }
+            }
+        }
     } else {
         // tile load: rows/quads that touch the world plus its one-cell WALL border
         const int ly_lo = max(0, -1 - gy0), ly_hi = min(LH, H + 1 - gy0);
@@ -896,6 +939,30 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
         return;
     }
     const int sy_hi = max(0, min(TH, H - (gy0 + HB))), sq_hi = max(0, min(TQ, (W - (gx0 + HB) + 3) >> 2));
+    if (SROWS) {  // tile at the world's edge: the rows mapping, cut to the world
+        const int lq = tid % QW, ly0 = tid / QW;
+        if (lq < sq_hi) {
+            const int gx = gx0 + HB + 4 * lq;
+            const u32* s = lds32 + (ly0 + HB) * LQ + HB / 4 + lq;
+#pragma unroll
+            for (int k = 0; k < KS; k++) {
+                const int ly = ly0 + k * DR;
+                if (ly < sy_hi) {
+                    const uint4 v = se_unpack_quad(s[k * DR * LQ]);
+                    u32* dst = out + (size_t)(gy0 + HB + ly) * pitch + gx;
+                    if (gx + 3 < W) {
+                        se_store_quad<true>(dst, v);
+                    } else {
+                        dst[0] = v.x;
+                        if (gx + 1 < W) dst[1] = v.y;
+                        if (gx + 2 < W) dst[2] = v.z;
+                    }
+                }
+            }
+        }
+        __syncthreads();  // LDS is reused by the next item of a persistent workgroup
+        return;
+    }
     for (int i = tid; i < sq_hi * sy_hi; i += NT) {
         const int ly = i / sq_hi, lq = i - ly * sq_hi;
         const int gx = gx0 + HB + 4 * lq, gy = gy0 + HB + ly;
