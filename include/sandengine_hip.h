@@ -126,7 +126,7 @@ int se_world_step_blocked(se_world* w, uint32_t nsteps, uint32_t sub);
  * b=tile h, c=max sub-steps * 1000 + threads per workgroup; kind 2: a=the LDS
  * variant se_world_step_blocked uses for `index` sub-steps per launch; kind 3:
  * a=workgroups of LDS variant `index` resident at once on the device (the
- * serving loop's persistent grid), b=its dynamic LDS bytes). */
+ * batch's persistent launch shape), b=its LDS bytes per workgroup). */
 int se_world_set_variant(se_world* w, int32_t plain_variant, int32_t lds_variant);
 int se_variant_info(const se_world* w, int32_t kind, int32_t index, int32_t* a, int32_t* b, int32_t* c);
 /* Render side-outputs of every step: the colour and light images that

@@ -16,8 +16,9 @@
 //     the next round (continuous batching, not a queue of whole runs);
 //   * retires a request when the round holding its last step completes, stamps
 //     its latency (arrival -> completion, host clock) and frees its arena space.
-// kernel 1 uses the batched persistent LDS kernel (one launch per rule set per
-// round); kernel 0 loops the plain kernel world by world inside each round.
+// kernel 1 uses the batched LDS kernel (one launch per rule set per round, one
+// workgroup per work item); kernel 0 loops the plain kernel world by world
+// inside each round.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -366,7 +367,10 @@ int enqueue_round(se_serve* S, Slot& sl, std::vector<size_t>& active, bool* any)
         int vi = lds_index(mod);
         const se::LdsVariant& v = mod->lds[vi].first;
         size_t b0 = ranges[gi].second.first, n = ranges[gi].second.second - b0;
-        rc = launch_ex(mod->batch[vi], (unsigned)mod->lds_grid[vi], (unsigned)v.nt, 0u, st,
+        // one workgroup per work item (measured faster than persistent workgroups:
+        // 2.51 against 2.15 Tcell/s processed under overload)
+        const unsigned items = (unsigned)sl.h_pre[pre_off[gi] + n];
+        rc = launch_ex(mod->batch[vi], items ? items : 1u, (unsigned)v.nt, 0u, st,
                        (const void*)((BWorld*)sl.d_recs + b0), (const int*)((int*)sl.d_pre + pre_off[gi]), (int)n, 0,
                        (int)(1 << 30), (const uint32_t*)mod->table);
         if (rc) return rc;
