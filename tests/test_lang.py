@@ -150,3 +150,22 @@ def test_exports_every_declared_symbol():
     lib = S.lib()
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def _defines(src):
+    return dict(re.findall(r"^#define (SE_\w+) (\S+)$", src, re.M))
+
+
+def test_hip_backend_picks_the_block_evaluation_form():
+    """Full transition table for the four-material set (6^4 tuples x 2 classes),
+    coded-table candidate for the twelve-material set (13^4 u8 entries, hash
+    lane 0 deciding only the mirror, 4 classes of lane 1), generated chain for
+    a set with a Right rule (its result is not a mirror of the mirrored one)."""
+    four = _defines(S.emit_hip(S.bundled_ruleset("four_materials")))
+    assert four["SE_TBL"] == "1" and four["SE_TBL_WORDS"] == str(6 ** 4 * 2) + "u"
+    twelve = _defines(S.emit_hip(S.bundled_ruleset("twelve_materials")))
+    assert twelve["SE_TBL"] == "2" and twelve["SE_CT_D"] == "13u" and twelve["SE_CT_R"] == "4u"
+    assert twelve["SE_CT_T1_WORDS"] == str((13 ** 4 + 15) // 16 * 4) + "u"
+    assert twelve["SE_CT_MIRROR_T"] == "0x7fffffbfu"  # rand.x <= 0.5 in the threshold form
+    feat = _defines(S.emit_hip(S.bundled_ruleset("features")))
+    assert feat["SE_TBL"] == "0" and feat["SE_TBL_WORDS"] == "0u"
