@@ -437,10 +437,14 @@ __device__ __forceinline__ u32 se_blk_read(const unsigned char* p0) {
 template <int LW, bool ODD>
 __device__ __forceinline__ void se_blk_write(unsigned char* p0, u32 e) {
     if (ODD) {
+        // bytes 0 and 2 from e, bytes 1 and 3 from e >> 8 (byte stores write
+        // bits 7:0 or 23:16 of a register)
+        u32 e8 = e >> 8;
+        __asm__("" : "+v"(e8));
         p0[0] = (unsigned char)e;
-        p0[1] = (unsigned char)(e >> 8);
+        p0[1] = (unsigned char)e8;
         p0[LW] = (unsigned char)(e >> 16);
-        p0[LW + 1] = (unsigned char)(e >> 24);
+        p0[LW + 1] = (unsigned char)(e8 >> 16);
     } else {
         *reinterpret_cast<unsigned short*>(p0) = (unsigned short)e;
         *reinterpret_cast<unsigned short*>(p0 + LW) = (unsigned short)(e >> 16);
@@ -498,6 +502,29 @@ __device__ __forceinline__ u32 se_ct_eval(const u32* __restrict__ lt, u32 t, con
     const u32 off = __umul24(v, SE_CT_R * 16u) + (se_rclass_ct(h[1], h[2], h[3]) * 16u + (mirror ? 8u : 0u));
     const uint2 o = *reinterpret_cast<const uint2*>(reinterpret_cast<const unsigned char*>(lt + SE_CT_T1_WORDS) + off);
     return __builtin_amdgcn_perm(o.y, t, o.x);
+}
+#endif
+
+#if SE_TBL == 1
+// Full table at odd x, where a block's cells come from LDS as four bytes: the
+// entry's byte offset tuple * 4R + 4r as 24-bit multiply-adds of the bytes
+// (no packing of the bytes into a word first).
+// a * b + c for a, b < 2^24 as one v_mad_u32_u24 (kept as a chain: the
+// compiler would otherwise split such sums into shifts, multiplies and adds)
+__device__ __forceinline__ u32 se_mad24(u32 a, u32 b, u32 c) {
+    u32 d;
+    __asm__("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+    return d;
+}
+
+template <int LW>
+__device__ __forceinline__ u32 se_tbl_at_bytes(const u32* __restrict__ ltbl, const unsigned char* p0, u32 r) {
+    constexpr u32 S = SE_TBL_R * 4u, N = SE_NMAT;
+    u32 off = se_mad24(p0[0], S, r * 4u);
+    off = se_mad24(p0[1], S * N, off);
+    off = se_mad24(p0[LW], S * N * N, off);
+    off = se_mad24(p0[LW + 1], S * N * N * N, off);
+    return *reinterpret_cast<const u32*>(reinterpret_cast<const unsigned char*>(ltbl) + off);
 }
 #endif
 
@@ -629,16 +656,26 @@ __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* 
             const int gya = gy0 + fy + 2 * by, gyb = gya + 2 * NW;
             unsigned char* pa = lds + (fy + 2 * by) * LW + lx;
             unsigned char* pb = two ? pa + 2 * NW * LW : pa;
-            const u32 ta = se_blk_read<LW, ODD>(pa), tb = RPI == 2 ? se_blk_read<LW, ODD>(pb) : 0u;
-            u32 ea, eb;
             const bool skipa = colb || (BORDER && (gya < 0 || gya + 1 >= H));
             const bool skipb = colb || (BORDER && (gyb < 0 || gyb + 1 >= H));
-#if SE_TBL
-            se_blk_eval2<FIXED>(ltbl, ta, tb, xa, kf, gya, gyb, skipa, skipb, ea, eb);
+            u32 ea, eb;
+#if SE_TBL == 1
+            if (ODD) {  // full table at odd x: offsets from the cell bytes
+                u32 ha[4], hb[4];
+                se_hash_lanes(ha, xa, kf, gya);
+                se_hash_lanes(hb, xa, kf, gyb);
+                ea = se_tbl_at_bytes<LW>(ltbl, pa, se_rclass(ha[0], ha[1], ha[2], ha[3]));
+                eb = se_tbl_at_bytes<LW>(ltbl, pb, se_rclass(hb[0], hb[1], hb[2], hb[3]));
+            } else {
+                const u32 ta = se_blk_read<LW, ODD>(pa), tb = se_blk_read<LW, ODD>(pb);
+                se_blk_eval2<FIXED>(ltbl, ta, tb, xa, kf, gya, gyb, skipa, skipb, ea, eb);
+            }
+#elif SE_TBL == 2
+            const u32 ta = se_blk_read<LW, ODD>(pa);
+            se_blk_eval2<FIXED>(ltbl, ta, 0u, xa, kf, gya, gyb, skipa, skipb, ea, eb);
 #else
-            (void)tb;
             eb = 0u;
-            ea = se_blk_eval1(ta, xa, kf, gya);
+            ea = se_blk_eval1(se_blk_read<LW, ODD>(pa), xa, kf, gya);
 #endif
             if (EDGE && !BORDER) {
                 ea = se_keep_wall(ea, colkeep, gya, H);
