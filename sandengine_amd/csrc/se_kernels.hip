@@ -289,6 +289,11 @@ __device__ __forceinline__ u32 se_enc(u32 v) { return se_enc_ok(se_decode(v)); }
 // tuple = c0 + N*(c1 + N*(c2 + N*c3)) and hash-threshold class r.  Built on the
 // device from se_block_h itself with one representative hash per class.
 // ---------------------------------------------------------------------------
+// 1 when no entry of the full table turns a WALL cell into something else (set
+// by the host after checking the built table): edge tiles then skip the
+// restoring of WALL outside the world, which the table already keeps.
+extern "C" __device__ u32 se_wall_kept = 0u;
+
 extern "C" __global__ __launch_bounds__(256) void se_build_table(u32* __restrict__ tbl) {
     const u32 tuple = blockIdx.x * 256u + threadIdx.x;
     if (tuple >= SE_TBL_ENTRIES / SE_TBL_R) return;
@@ -850,6 +855,13 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     if (SE_TBL == 2 && fixed) *flag = seq;
     __syncthreads();
     bool tile_fixed = SE_TBL == 2 && *flag == seq;
+    // edge tiles need the WALL restoring of their block loop unless the full
+    // table keeps WALL cells by itself
+#if SE_TBL == 1
+    const bool mask_wall = !interior && se_wall_kept == 0u;
+#else
+    const bool mask_wall = !interior;
+#endif
     for (int s = 0; s < nsub; s++) {
         const u32 f = (u32)(frame0 + s);
         int ox, oy;
@@ -860,7 +872,7 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
         const int nb = nbx * nby;
         const bool use_mods = s == 0 && nmods > 0;
         if (!use_mods) {
-            se_lds_substep<LW, NT>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f, !interior, tile_fixed);
+            se_lds_substep<LW, NT>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f, mask_wall, tile_fixed);
         } else if (nb > 0) {
             const int dby = NT / nbx, dbx = NT - dby * nbx;
             int by = tid / nbx, bx = tid - by * nbx;

@@ -253,6 +253,22 @@ int build_ctable(se_ctx* ctx, const se::ParsingResult& pr, const se::HipGenInfo&
 
 namespace {
 
+// Does every entry of a full transition table (entry tuple * R + r holds the
+// block result as four id bytes; tuple = c0 + n*(c1 + n*(c2 + n*c3))) leave each
+// WALL cell of its tuple WALL?
+bool table_keeps_wall(const std::vector<uint32_t>& tbl, size_t n, size_t R) {
+    const uint32_t wall = 2u;
+    for (size_t tuple = 0; tuple < n * n * n * n; tuple++) {
+        uint32_t mask = 0, t = (uint32_t)tuple;
+        for (int i = 0; i < 4; i++, t /= (uint32_t)n)
+            if (t % n == wall) mask |= 0xffu << (8 * i);
+        if (!mask) continue;
+        for (size_t r = 0; r < R; r++)
+            if ((tbl[tuple * R + r] & mask) != ((wall * 0x01010101u) & mask)) return false;
+    }
+    return true;
+}
+
 // ctwords: the compacted coded table when the source was generated for one.
 int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
                 const std::vector<uint32_t>* ctwords, std::shared_ptr<se_module>& out) {
@@ -297,6 +313,20 @@ int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
         int rc = launch(build, (unsigned)((tuples + 255) / 256), ctx->stream, m->table);
         if (rc) return rc;
         HIPCHECK(hipStreamSynchronize(ctx->stream));
+        // Edge tiles hold WALL for the cells outside the world.  If no entry
+        // changes a WALL cell, the table keeps them by itself and the edge-tile
+        // block loop can skip restoring them (se_wall_kept).
+        std::vector<uint32_t> tbl(m->table_entries);
+        HIPCHECK(hipMemcpy(tbl.data(), m->table, tbl.size() * 4, hipMemcpyDeviceToHost));
+        m->wall_kept = table_keeps_wall(tbl, n, (size_t)info.table_R);
+        if (m->wall_kept) {
+            void* flag = nullptr;
+            size_t flag_bytes = 0;
+            HIPCHECK(hipModuleGetGlobal(&flag, &flag_bytes, m->mod, "se_wall_kept"));
+            if (flag_bytes != sizeof(uint32_t)) return fail(SE_ERR_JIT, "internal: se_wall_kept has the wrong size");
+            const uint32_t one = 1u;
+            HIPCHECK(hipMemcpy(flag, &one, sizeof one, hipMemcpyHostToDevice));
+        }
     }
     if (info.ctable) {
         if (!ctwords) return fail(SE_ERR_JIT, "internal: coded-table source without its table");

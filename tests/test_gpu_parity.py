@@ -171,6 +171,47 @@ def test_coded_table_fallback_vs_oracle(shape):
         np.testing.assert_array_equal(got, want, err_msg=f"{mode} sub={sub}")
 
 
+def _erode_rules():
+    """The four-material rule set plus a rule that replaces WALL (sand sets the
+    WALL below it to water) and one that then sinks sand into water.  At the
+    world's bottom border the first writes water into the WALL cell outside the
+    world (dropped there, as in the reference) and the second swaps the sand
+    into it, so the sand turns into water.  Entries of the full table change WALL
+    cells, so edge tiles must restore the WALL outside the world after every
+    block."""
+    t = S.bundled_ruleset("four_materials")
+    t2 = t.replace("    extra_rules: [sand_fall]\n", "    extra_rules: [sand_sink, sand_erode, sand_fall]\n", 1)
+    t2 = t2.replace("types:\n", "  sand_erode:\n    if: DOWN.mat.type == TYPE_WALL\n    probability: 0.5\n"
+                                 "    do: SET DOWN water\n"
+                                 "  sand_sink:\n    if: DOWN.mat == water\n    do: SWAP SELF DOWN\ntypes:\n", 1)
+    assert t2 != t
+    return t2
+
+
+@pytest.mark.parametrize("shape", [(300, 260), (131, 77)], ids=["300x260", "131x77"])
+def test_wall_changing_table_vs_oracle(shape):
+    """A full-table rule set whose entries change WALL cells steps exactly in the
+    LDS kernel's edge tiles (which then restore the WALL border); the bundled
+    four-material set, whose table keeps WALL, skips that restoring."""
+    W, H = shape
+    print(f"\n[gpu] erode {W}x{H}", flush=True)
+    text = _erode_rules()
+    rs = S.RuleSet(ctx(), text)
+    assert rs.kernel_form == "table"
+    blob = O.compile_ruleset(L.parse_string(text))
+    rng = np.random.RandomState(W + 31 * H)
+    g0 = O.fill(int(rng.randint(0, 10 ** 6)), W, H, [0, 3, 4, 5], [50, 25, 15, 10])
+    steps, first = 23, 9
+    mods_at = {5: C._rand_mods(rng, W, H, 6, 3)}
+    want = g0
+    for s in range(steps):
+        want = O.step(blob, want, first + s, mods_at.get(s))
+    assert (want == 3).sum() < (g0 == 3).sum()  # sand turned into water at the border
+    for mode, sub in [("plain", 0), ("blocked", 0), ("blocked", 6)]:
+        got, _, _ = run_protocol(rs, g0, first, steps, mods_at, mode, sub)
+        np.testing.assert_array_equal(got, want, err_msg=f"{mode} sub={sub}")
+
+
 def test_clear_frame_and_api_errors():
     rs = ruleset("four")
     w = S.World(ctx(), rs, 40, 30)
