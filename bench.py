@@ -457,7 +457,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": R["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": R["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": 8 * cells, "avg_launch_ms": R["avg_launch_ms"],
-                     "binding_limit": "hbm" if best == "plain" else "valu (see roofline_valu)"},
+                     "binding_limit": "hbm" if best == "plain" else "valu issue and lds latency (see roofline_valu)"},
         "roofline_valu": valu,
         "cpu_baseline": cpu,
         "copy_reference": {"gbs": copy_gbs, "ms": copy_ms, "frac_of_peak": copy_gbs / HBM_PEAK_GBS},
