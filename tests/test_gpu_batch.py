@@ -114,3 +114,29 @@ def test_batch_small_worlds_against_oracle():
     for w, g0, steps in cases:
         np.testing.assert_array_equal(w.download(), O.run(blob, g0, 2, steps))
     b.close()
+
+
+def test_batch_wall_changing_table_against_oracle():
+    """A full-table rule set whose entries change WALL cells (sand sets the WALL
+    below it to water): the batched kernel's edge tiles restore the world's WALL
+    border, in single-tile worlds and in worlds that need rounds."""
+    from tests.test_gpu_parity import _erode_rules
+    text = _erode_rules()
+    rset = S.RuleSet(ctx(), text)
+    assert rset.kernel_form == "table"
+    blob = O.compile_ruleset(L.parse_string(text))
+    b = S.Batch(ctx())
+    cases = []
+    for i, (W, H, steps) in enumerate([(64, 64, 120), (150, 90, 77), (300, 260, 41), (31, 17, 200)]):
+        g0 = O.fill(70 + i, W, H, [0, 3, 4, 5], [50, 25, 15, 10])
+        w = S.World(ctx(), rset, W, H)
+        w.frame = 1
+        w.upload(g0)
+        b.add(w, steps)
+        cases.append((w, g0, steps))
+    b.run(mode=1, graph=True)
+    for w, g0, steps in cases:
+        want = O.run(blob, g0, 2, steps)
+        assert (want == 3).sum() < (g0 == 3).sum()
+        np.testing.assert_array_equal(w.download(), want)
+    b.close()
