@@ -65,47 +65,118 @@ def batch_specs(name):
     return specs[:n]
 
 
+SERVE_MIX = dict(side_min=128, side_max=4096, steps_min=10, steps_max=5000, mods_max=20)
+
+
+def replay_alone(S, ctx, rsets, fills, r):
+    """FNV-1a 64 of a served request's final grid when the same request runs
+    alone on the plain kernel (same fill, same modifications at the same steps)."""
+    ids, pct = fills[r["ruleset"]]
+    w = S.World(ctx, rsets[r["ruleset"]], r["side"], r["side"])
+    w.frame = 1
+    w.fill_seeded(r["fill_seed"], ids, pct)
+    by_step = {}
+    for step, m in r["mods"]:
+        by_step.setdefault(step, []).append(m)
+    done = 0
+    for st in sorted(by_step) + [r["steps"]]:
+        if st > done:
+            w.step(st - done)
+            done = st
+        if st < r["steps"]:
+            for m in by_step[st]:
+                w.push_modification(*m)
+    h = S.fnv1a64(w.download())
+    w.close()
+    return h
+
+
+def serve_once(S, ctx, rsets, fills, args, kernel, rate, duration, drain, slo_ms, verify_every):
+    srv = S.Serve(ctx, rsets, fills, rate=rate, duration_s=duration, seed=4, kernel=kernel, max_inflight=2,
+                  drain_timeout_s=drain, hbm_budget=int(args.budget_gb * (1 << 30)), verify_every=verify_every,
+                  slo_ms=slo_ms, **SERVE_MIX)
+    t0 = time.perf_counter()
+    st = srv.run()
+    st.update(rate=rate, duration_s=duration, drain_s=drain, slo_ms=slo_ms or None,
+              offered_over_processed=st["offered_gcups"] / max(st["processed_gcups"], 1e-9),
+              censored=st["unfinished"] > 0)
+    log(f"kernel {kernel} rate {rate}/s {duration}s slo {slo_ms or '-'}: {st['completed']}/{st['requests']} done, "
+        f"{st['shed']} shed ({st['shed_cell_frac']:.2f} of the work), {st['unfinished']} unfinished, "
+        f"p50 {st['p50_ms']:.0f} ms, p99 {st['p99_ms']:.0f} ms, {st['processed_gcups']:.0f} Gcell/s processed "
+        f"({st['gcups']:.0f} completed) vs {st['offered_gcups']:.0f} offered, censored p99 "
+        f"{st['p99_ms_censored']:.0f} ms, capacity est {st['capacity_gcups_est']:.0f} Gcell/s, round fixed "
+        f"{st['round_fixed_ms']:.3f} ms, actual/predicted latency p50 {st['pred_ratio_p50']:.2f} p99 "
+        f"{st['pred_ratio_p99']:.2f}, peak {st['peak_bytes'] / 2**30:.1f} GiB, {time.perf_counter() - t0:.1f}s")
+    if verify_every:
+        t1 = time.perf_counter()
+        n = eq = 0
+        for i in range(len(srv)):
+            r = srv.request(i)
+            if r["verified"]:
+                n += 1
+                eq += int(replay_alone(S, ctx, rsets, fills, r) == r["hash"])
+        st["verified"] = {"every": verify_every, "requests": n, "equal_alone": eq,
+                          "check": "final grid (FNV-1a 64) equals the request run alone on the plain kernel"}
+        log(f"verified {eq}/{n} requests equal to their run alone ({time.perf_counter() - t1:.1f}s)")
+    reqs = [srv.request(i) for i in range(min(len(srv), 64))]
+    srv.close()
+    return st, reqs
+
+
 def run_serve(args):
     """Serving loop (Poisson arrivals, HBM-budget admission, continuous batching):
-    p50/p99 request latency and aggregate throughput, batched LDS kernel vs the
-    plain kernel looped per world."""
+    p50/p99 request latency and aggregate throughput.  Runs, all on the served mix:
+    the batched LDS kernel with FIFO admission at --rate; the same with deadline
+    admission (--slo-ms); an arrival-rate sweep of both; the plain kernel looped
+    per world at --plain-rate (censored when it cannot keep up)."""
     import sandengine_amd as S
     ctx = S.Context(0)
-    rsets = [S.RuleSet(ctx, S.bundled_ruleset(k)) for k in BATCH_FILLS]
-    fills = [BATCH_FILLS[k] for k in BATCH_FILLS]
-    out_runs = {}
-    for kernel in [int(k) for k in args.serve_kernels.split(",")]:
-        rate = args.rate if kernel == 1 else args.plain_rate
-        dur = args.duration if kernel == 1 else args.plain_duration
-        srv = S.Serve(ctx, rsets, fills, rate=rate, duration_s=dur, seed=4, side_min=128, side_max=4096,
-                      steps_min=10, steps_max=5000, mods_max=20, kernel=kernel, max_inflight=2,
-                      drain_timeout_s=args.drain, hbm_budget=int(args.budget_gb * (1 << 30)), verify_every=0)
-        t0 = time.perf_counter()
-        st = srv.run()
-        st["rate"] = rate
-        st["duration_s"] = dur
-        st["offered_over_processed"] = st["offered_gcups"] / max(st["processed_gcups"], 1e-9)
-        name = "batched_lds" if kernel == 1 else "plain_looped"
-        out_runs[name] = st
-        log(f"{name}: rate {rate}/s for {dur}s: {st['completed']}/{st['requests']} done "
-            f"({st['unfinished']} unfinished), p50 {st['p50_ms']:.0f} ms, p99 {st['p99_ms']:.0f} ms, "
-            f"{st['processed_gcups']:.0f} Gcell-upd/s processed ({st['gcups']:.0f} in completed requests) vs "
-            f"{st['offered_gcups']:.0f} offered, censored p50/p99 {st['p50_ms_censored']:.0f}/"
-            f"{st['p99_ms_censored']:.0f} ms, rounds {st['rounds']}, "
-            f"peak {st['peak_bytes'] / 2**30:.1f} GiB, {time.perf_counter() - t0:.1f}s")
-        srv.close()
-    best = out_runs.get("batched_lds") or next(iter(out_runs.values()))
+    names = list(BATCH_FILLS)
+    rsets = [S.RuleSet(ctx, S.bundled_ruleset(k)) for k in names]
+    fills = [BATCH_FILLS[k] for k in names]
+    runs, sweep = {}, []
+    ver = args.verify
+    runs["batched_fifo"], reqs = serve_once(S, ctx, rsets, fills, args, 1, args.rate, args.duration, args.drain, 0, ver)
+    if args.slo_ms > 0:
+        runs["batched_deadline"], _ = serve_once(S, ctx, rsets, fills, args, 1, args.rate, args.duration, args.drain,
+                                                 args.slo_ms, ver)
+    for rate in [float(x) for x in args.sweep.split(",") if x]:
+        for slo in ([0.0, args.slo_ms] if args.slo_ms > 0 else [0.0]):
+            st, _ = serve_once(S, ctx, rsets, fills, args, 1, rate, args.sweep_duration, args.sweep_drain, slo, 0)
+            sweep.append({k: st[k] for k in ("rate", "slo_ms", "requests", "completed", "shed", "unfinished",
+                                              "p50_ms", "p99_ms", "p99_ms_censored", "slo_met_frac",
+                                              "shed_cell_frac", "processed_gcups", "gcups", "offered_gcups",
+                                              "offered_over_processed", "censored")})
+    if args.plain_duration > 0:
+        runs["plain_looped"], _ = serve_once(S, ctx, rsets, fills, args, 0, args.plain_rate, args.plain_duration,
+                                             args.plain_drain, 0, 0)
+    nb = min(16, len(reqs))
+    cpu = None if args.no_cpu_baseline else cpu_baseline(
+        [(names[r["ruleset"]], r["side"], r["fill_seed"], fills[r["ruleset"]][0], fills[r["ruleset"]][1], 0, 0)
+         for r in reqs[:nb]], steps=8, warm=1,
+        what=f"offline: the first {nb} requests of the same arrival sequence (sides, rule sets, fills), one after "
+             f"another")
+    best = runs["batched_fifo"]
     out = {
         "metric": "Gcell-updates/s & HBM-roofline %; p50/p99 step latency, batched vs plain",
         "value": best["processed_gcups"], "unit": "Gcell-updates/s", "n_gpus": 1, "steps": None, "warmup": 0,
         "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic (seeded Poisson arrivals and SplitMix64 fills)",
-        "config": {"workload": "serving: Poisson arrivals, side log-uniform [128,4096], 4- or 12-material rules, "
-                               "steps log-uniform [10,5000], 0-20 circle modifications (brush r 4-64 / "
-                               "explosion r 8-128), seed 4, HBM admission budget %.0f GiB" % args.budget_gb},
-        "runs": out_runs,
+        "config": {"workload": "serving: Poisson arrivals at %g/s for %g s, side log-uniform [128,4096], 4- or "
+                               "12-material rules, steps log-uniform [10,5000], 0-20 circle modifications (brush "
+                               "r 4-64 / explosion r 8-128), seed 4, HBM admission budget %.0f GiB"
+                               % (args.rate, args.duration, args.budget_gb),
+                   "value_is": "batched_fifo processed Gcell-updates/s (all steps run over the wall time)",
+                   "slo_ms": args.slo_ms or None},
+        "runs": runs,
+        "rate_sweep": sweep,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+    bad = [k for k, v in runs.items() if isinstance(v.get("verified"), dict)
+           and v["verified"]["equal_alone"] != v["verified"]["requests"]]
+    if bad:
+        sys.exit(1)
 
 
 def run_batch(args):
@@ -159,6 +230,11 @@ def run_batch(args):
                 identical = False
                 break
         log(f"batched == looped plain for every world: {identical}")
+    nb = min(64, len(specs))
+    cpu = None if args.no_cpu_baseline else cpu_baseline(
+        [(rs, side, fseed, BATCH_FILLS[rs][0], BATCH_FILLS[rs][1], 0, 0) for side, rs, _, fseed in specs[:nb]],
+        what=f"the first {nb} worlds of the same draw, one after another (one Simulation per world, as the "
+             f"reference runs them)")
     out = {
         "metric": "Gcell-updates/s & HBM-roofline %; p50/p99 step latency, batched vs plain",
         "value": cells / (wall_batch) / 1e9,
@@ -173,6 +249,7 @@ def run_batch(args):
         "looped_plain": {"gpu_ms": ms_loop, "wall_s": wall_loop, "gcups_gpu": cells / (ms_loop / 1e3) / 1e9,
                          "gcups_wall": cells / wall_loop / 1e9},
         "speedup_batched_vs_looped_plain": wall_loop / wall_batch,
+        "cpu_baseline": cpu,
         "cell_updates": cells,
         "parity": {"identical": identical},
     }
@@ -224,7 +301,9 @@ def run_render(args):
     kernel = "se_render"
     prof = load_profile(kernel, args.workload)
     achieved = RENDER_BYTES_PER_CELL * cells / (render_ms / 1e3) / 1e9
-    cpu = None if args.no_cpu_baseline else cpu_baseline("single16k")
+    cpu = None if args.no_cpu_baseline else cpu_baseline(
+        single_sample("single16k"), what="1024x1024 world of the same fill and rules as single16k (the reference "
+                                         "program computes colour and light in every step too)")
     out = {
         "metric": "Gcell-updates/s & HBM-roofline %; p50/p99 step latency, batched vs plain",
         "value": cells * args.steps / wall / 1e9, "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -281,43 +360,86 @@ def latencies(w, mode, sub, n):
     return np.percentile(out, 50), np.percentile(out, 99)
 
 
-def cpu_baseline(wl, side=1024, steps=12, warm=2):
-    """Bounded sample of the same workload on the host cores, in a child process."""
+def host_info():
+    """The host the CPU baseline runs on: logical CPUs, the CPUs this process may
+    use, and the CPU model."""
+    model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "cpus_allowed": allowed, "cpu_model": model}
+
+
+# Host threads for the CPU baseline: the CPUs this process may use, at most 16
+# (the host share of one GPU on the measurement machines).
+BASELINE_THREADS = max(1, min(16, host_info()["cpus_allowed"]))
+
+
+def cpu_baseline(samples, steps=12, warm=2, what=""):
+    """The reference's GLSL program under Mesa llvmpipe (or, where that build is
+    absent, the oracle's C restatement) on a bounded sample of the workload, in a
+    child process: `samples` are worlds (rule set, side, fill seed, ids,
+    percents, stone rows, stone id), each run for `warm` untimed and `steps`
+    timed steps; the value is their cell-updates over the summed timed runs."""
     code = f"""
 import json, os, sys, time
 sys.path.insert(0, {ROOT!r})
 import numpy as np
 from oracle import lang_ref as L, oracle as O, llvmpipe as LP
-side, steps, warm = {side}, {steps}, {warm}
-_, rsname, seed, ids, pct, (num, den), stone = {WORKLOADS[wl]!r}
-text = open(os.path.join({ROOT!r}, 'sandengine_amd', 'rulesets', rsname + '.yaml')).read()
-res = L.parse_string(text)
-g = O.fill(seed, side, side, ids, pct, (side * num) // den, stone)
-cores = min(16, os.cpu_count() or 1)
-if LP.available():
-    LP.lib(cores)
-    w = LP.World(LP.program_for(res), side, side)
-    w.run(1); w.upload(g)
-    w.run(warm)
-    dt = w.time_run(steps)
-    kind, extra = 'reference', LP.info()
+samples, steps, warm, threads = {samples!r}, {steps}, {warm}, {BASELINE_THREADS}
+progs, blobs, cells, dt = {{}}, {{}}, 0.0, 0.0
+use_lp = LP.available()
+if use_lp:
+    LP.lib(threads)
 else:
-    blob = O.compile_ruleset(res)
-    os.environ['OMP_NUM_THREADS'] = str(cores)
-    g = O.run(blob, g, 2, warm)
-    t = time.perf_counter(); O.run(blob, g, 2 + warm, steps); dt = time.perf_counter() - t
-    kind, extra = 'port', 'oracle C restatement (OpenMP)'
-print(json.dumps(dict(value=side * side * steps / dt / 1e9, unit='Gcell-updates/s', cores=cores, kind=kind,
-                      sample=f'{{side}}x{{side}} world of the same fill and rules, {{steps}} timed steps after '
-                             f'{{warm}} warm-up; {{extra}}')))
+    os.environ['OMP_NUM_THREADS'] = str(threads)
+for rsname, side, seed, ids, pct, stone_rows, stone_id in samples:
+    if rsname not in progs:
+        res = L.parse_string(open(os.path.join({ROOT!r}, 'sandengine_amd', 'rulesets', rsname + '.yaml')).read())
+        progs[rsname] = LP.program_for(res) if use_lp else None
+        blobs[rsname] = O.compile_ruleset(res)
+    g = O.fill(seed, side, side, ids, pct, stone_rows, stone_id)
+    if use_lp:
+        w = LP.World(progs[rsname], side, side)
+        w.run(1); w.upload(g)
+        w.run(warm)
+        dt += w.time_run(steps)
+    else:
+        g = O.run(blobs[rsname], g, 2, warm)
+        t = time.perf_counter(); O.run(blobs[rsname], g, 2 + warm, steps); dt += time.perf_counter() - t
+    cells += float(side) * side * steps
+kind, extra = ('reference', LP.info()) if use_lp else ('port', 'oracle C restatement (OpenMP)')
+print(json.dumps(dict(value=cells / dt / 1e9, unit='Gcell-updates/s', cores=threads, kind=kind, seconds=dt,
+                      engine=extra)))
 """
     env = dict(os.environ)
-    env["LP_NUM_THREADS"] = str(min(16, os.cpu_count() or 1))
-    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=600)
+    env["LP_NUM_THREADS"] = str(BASELINE_THREADS)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=900)
     if r.returncode != 0:
         log("cpu baseline failed:", r.stderr[-2000:])
         return None
-    return json.loads(r.stdout.strip().splitlines()[-1])
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out["sample"] = (f"{what}; {warm} untimed + {steps} timed steps per world; reference dispatch geometry "
+                     f"((W+7)/8, (H+7)/8) x 16x16; {out.pop('engine')}")
+    out["host"] = dict(host_info(), threads_used=BASELINE_THREADS,
+                       note="LP_NUM_THREADS = threads_used: the CPUs allowed to this process, at most 16 (the "
+                            "host share of one GPU)")
+    log(f"cpu baseline: {out['value']:.4f} Gcell-upd/s on {BASELINE_THREADS} threads ({out['seconds']:.1f} s timed)")
+    return out
+
+
+def single_sample(wl, side=1024):
+    """A side x side world of the same rules and fill as workload `wl`."""
+    _, rsname, seed, ids, pct, (num, den), stone = WORKLOADS[wl]
+    return [(rsname, side, seed, ids, pct, (side * num) // den, stone)]
 
 
 def load_profile(kernel_prefix, workload):
@@ -350,17 +472,23 @@ def main():
                     choices=sorted(WORKLOADS) + sorted(BATCH) + ["serve", "render16k"])
     ap.add_argument("--tune-rounds", type=int, default=2)
     ap.add_argument("--rate", type=float, default=2000.0, help="serve: requests/s (batched kernel)")
-    ap.add_argument("--duration", type=float, default=60.0, help="serve: arrival window (s)")
-    ap.add_argument("--plain-rate", type=float, default=200.0)
-    ap.add_argument("--plain-duration", type=float, default=10.0)
+    ap.add_argument("--duration", type=float, default=20.0, help="serve: arrival window (s)")
     ap.add_argument("--drain", type=float, default=30.0, help="serve: stop this long after the last arrival")
+    ap.add_argument("--slo-ms", type=float, default=1000.0, help="serve: latency objective of deadline admission")
+    ap.add_argument("--sweep", default="800,1200,1600,2000", help="serve: arrival rates of the sweep")
+    ap.add_argument("--sweep-duration", type=float, default=10.0)
+    ap.add_argument("--sweep-drain", type=float, default=20.0)
+    ap.add_argument("--plain-rate", type=float, default=2000.0)
+    ap.add_argument("--plain-duration", type=float, default=5.0)
+    ap.add_argument("--plain-drain", type=float, default=5.0)
+    ap.add_argument("--verify", type=int, default=0, help="serve: replay every n-th request alone (0: none)")
     ap.add_argument("--budget-gb", type=float, default=200.0)
-    ap.add_argument("--serve-kernels", default="1,0")
     ap.add_argument("--sub", type=int, default=8, help="sub-steps per LDS launch")
     ap.add_argument("--paths", default="plain,blocked")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--latency-samples", type=int, default=50)
+    ap.add_argument("--roofline-launches", type=int, default=10)
     ap.add_argument("--copy-reps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -402,13 +530,22 @@ def main():
                  ms_per_step=wall * 1e3 / args.steps, avg_launch_ms=avg_launch_ms, launches=launches,
                  steps_per_launch=1 if mode == "plain" else args.sub,
                  kernel="se_step_plain_r1_nt1" if mode == "plain" else lds_kernel)
-        r["achieved_gbs"] = 8.0 * cells / (avg_launch_ms / 1e3) / 1e9
+        # roofline timing: launches that each run the full `sub` sub-steps (the timed
+        # steps may end on a shorter launch), between HIP events on the world's stream
+        if mode == "blocked":
+            w.step_timed(2 * args.sub, 1, args.sub)
+            full_ms = w.step_timed(args.sub * args.roofline_launches, 1, args.sub) / args.roofline_launches
+        else:
+            full_ms = avg_launch_ms
+        r["full_launch_ms"] = full_ms
+        r["achieved_gbs"] = 8.0 * cells / (full_ms / 1e3) / 1e9
         r["roofline_frac"] = r["achieved_gbs"] / HBM_PEAK_GBS
         r["frac_of_copy"] = r["achieved_gbs"] / copy_gbs
         p50, p99 = latencies(w, mode, args.sub, args.latency_samples)
         r["step_latency_ms"] = {"p50": p50, "p99": p99}
         log(f"{mode}: {r['gcups']:.1f} Gcell-upd/s wall, {r['gcups_gpu']:.1f} by events, "
-            f"{r['avg_launch_ms']:.3f} ms/launch, {r['achieved_gbs']:.0f} GB/s algorithmic")
+            f"{r['avg_launch_ms']:.3f} ms/launch in the timed steps, {full_ms:.3f} ms per full launch, "
+            f"{r['achieved_gbs']:.0f} GB/s algorithmic")
         results[mode] = r
         worlds[mode] = w
     check = None
@@ -433,12 +570,13 @@ def main():
     traffic = prof.get("bytes_per_launch")
     valu = None
     if prof.get("valu_wave_instr_per_launch"):
-        rate = prof["valu_wave_instr_per_launch"] / (R["avg_launch_ms"] / 1e3)
+        rate = prof["valu_wave_instr_per_launch"] / (R["full_launch_ms"] / 1e3)
         valu = {"bound": "valu", "kernel": kernel, "achieved": rate, "peak": VALU_PEAK_WAVE_INSTR_PER_S,
                 "unit": "wave64 VALU instructions/s", "frac": rate / VALU_PEAK_WAVE_INSTR_PER_S,
                 "instr_per_launch": prof["valu_wave_instr_per_launch"],
                 "source": "SQ_INSTS_VALU per launch from profiles/traffic.json over this run's launch time"}
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(
+        single_sample(args.workload), what=f"1024x1024 world of the same fill and rules as {args.workload}")
     out = {
         "metric": "Gcell-updates/s & HBM-roofline %; p50/p99 step latency, batched vs plain",
         "value": R["gcups"],
@@ -456,7 +594,10 @@ def main():
                    "steps_per_launch": R["steps_per_launch"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": R["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": R["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": 8 * cells, "avg_launch_ms": R["avg_launch_ms"],
+                     "algorithmic_bytes_per_launch": 8 * cells, "avg_launch_ms": R["full_launch_ms"],
+                     "launch_timing": ("the timed steps' launches" if best == "plain" else
+                                       f"{args.roofline_launches} launches of {args.sub} sub-steps after the timed "
+                                       f"steps"),
                      "binding_limit": "hbm" if best == "plain" else "valu issue and lds latency (see roofline_valu)"},
         "roofline_valu": valu,
         "cpu_baseline": cpu,

@@ -219,7 +219,14 @@ int se_batch_tuned(const se_batch* b, const se_ruleset* rs, int32_t* variant, ui
  * fill, a step count and circle modifications (brushes of a random material,
  * explosions of EMPTY) at given steps.  Admission is FIFO by HBM budget (one
  * arena, no allocation per request); admitted worlds advance together in rounds
- * (continuous batching) and each result equals the world stepped alone. */
+ * (continuous batching) and each result equals the world stepped alone.
+ * With slo_ms > 0 admission is also by deadline: the loop predicts each
+ * request's completion from the rounds it needs and the round time (a fixed
+ * part plus the round's cell-updates over a capacity learned online from the
+ * rounds' GPU time); a queued request is admitted only when it and every
+ * admitted request are predicted to finish within arrival + 0.85 slo_ms (the
+ * rest is headroom for host delays and model error), waits while it could
+ * still make that, and is shed (state 4) once it cannot. */
 typedef struct {
     double rate;              /* requests per second */
     double duration_s;        /* arrival window */
@@ -234,6 +241,8 @@ typedef struct {
     double drain_timeout_s;   /* stop this long after the last arrival (0 = run to completion) */
     size_t hbm_budget;        /* bytes of world state admitted at once */
     uint32_t verify_every;    /* keep a hash of every n-th request's final grid (0 = none) */
+    double slo_ms;            /* deadline admission (see above); 0 = FIFO by HBM budget only */
+    double capacity_gcups;    /* starting value of the online capacity estimate; 0 = 1500 */
 } se_serve_config;
 
 typedef struct {
@@ -244,13 +253,20 @@ typedef struct {
     size_t peak_bytes;                      /* arena high-water mark */
     double processed_gcups;                 /* all steps run (also of unfinished requests) over wall */
     double p50_ms_censored, p99_ms_censored;/* over every arrived request, unfinished ones at the end time */
+    uint64_t shed, verified;                /* requests shed by deadline admission; final grids hashed */
+    double shed_cell_frac;                  /* shed requests' share of the offered cell-updates */
+    double slo_met_frac;                    /* completed requests done within slo_ms (1 when slo_ms is 0) */
+    double capacity_gcups_est;              /* the capacity estimate at the end of the run */
+    double round_fixed_ms;                  /* the fixed part of the round-time model */
+    double pred_ratio_p50, pred_ratio_p99;  /* completed requests' latency over the latency predicted at
+                                               admission (deadline admission; 0 otherwise) */
 } se_serve_stats;
 
 typedef struct {
     double arrival_s, done_s;
     uint32_t side, steps, ruleset, nmods;
     uint64_t fill_seed;
-    int state;    /* 0 waiting, 1 admitted, 2 done, 3 rejected (larger than the budget) */
+    int state;    /* 0 waiting, 1 admitted, 2 done, 3 rejected (larger than the budget), 4 shed (deadline) */
     int verified; /* hash holds FNV-1a 64 of the final grid (row-major u32) */
     uint64_t hash;
 } se_serve_request;
@@ -260,6 +276,7 @@ int se_serve_create(se_ctx* ctx, const se_ruleset* const* rulesets, int n_rulese
 void se_serve_destroy(se_serve* s);
 /* The i.i.d. fill of requests that use rule set `rs` (see se_world_fill_seeded). */
 int se_serve_set_fill(se_serve* s, int rs, const uint32_t* ids, const uint32_t* thresholds, int n);
+/* Runs the arrival window (and the drain); once per se_serve (SE_ERR_STATE after). */
 int se_serve_run(se_serve* s, se_serve_stats* stats);
 uint64_t se_serve_request_count(const se_serve* s);
 int se_serve_request_info(const se_serve* s, uint64_t i, se_serve_request* out);

@@ -449,7 +449,7 @@ class ServeConfig(ctypes.Structure):
                 ("steps_max", ctypes.c_uint32), ("mods_max", ctypes.c_uint32), ("quantum", ctypes.c_uint32),
                 ("quantum_small", ctypes.c_uint32), ("kernel", ctypes.c_int), ("max_inflight", ctypes.c_int),
                 ("drain_timeout_s", ctypes.c_double), ("hbm_budget", ctypes.c_size_t),
-                ("verify_every", ctypes.c_uint32)]
+                ("verify_every", ctypes.c_uint32), ("slo_ms", ctypes.c_double), ("capacity_gcups", ctypes.c_double)]
 
 
 class ServeStats(ctypes.Structure):
@@ -459,7 +459,10 @@ class ServeStats(ctypes.Structure):
                 ("wall_s", ctypes.c_double), ("cell_updates", ctypes.c_double), ("gcups", ctypes.c_double),
                 ("offered_gcups", ctypes.c_double), ("peak_bytes", ctypes.c_size_t),
                 ("processed_gcups", ctypes.c_double), ("p50_ms_censored", ctypes.c_double),
-                ("p99_ms_censored", ctypes.c_double)]
+                ("p99_ms_censored", ctypes.c_double), ("shed", ctypes.c_uint64), ("verified", ctypes.c_uint64),
+                ("shed_cell_frac", ctypes.c_double), ("slo_met_frac", ctypes.c_double),
+                ("capacity_gcups_est", ctypes.c_double), ("round_fixed_ms", ctypes.c_double),
+                ("pred_ratio_p50", ctypes.c_double), ("pred_ratio_p99", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -5,6 +5,11 @@
 //   * admits arrived requests in FIFO order while their state fits the HBM
 //     budget: both grids come from one pre-allocated arena (no hipMalloc per
 //     request) and the seeded fill is launched asynchronously;
+//   * with a latency objective (slo_ms), admits a request only when the round
+//     model predicts that it and every admitted request finish within their
+//     deadlines, lets it wait while it still could, and sheds it once it cannot
+//     (the round model: a fixed part plus the round's cell-updates over a
+//     capacity, both learned from the rounds' GPU time);
 //   * every round advances each admitted world by a quantum (at most the tile
 //     halo for worlds spanning several LDS tiles, `quantum_small` for worlds
 //     that fit one tile), cut so that a world's modifications always fall on the
@@ -22,6 +27,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <deque>
 #include <thread>
 
@@ -125,10 +131,11 @@ struct Request {
     double done_t = 0;
     int verified = 0;
     uint64_t hash = 0;
+    double pred_ms = 0;  // latency predicted at admission (deadline admission)
 };
 
 struct Slot {  // one in-flight round
-    hipEvent_t ev = nullptr;
+    hipEvent_t ev0 = nullptr, ev = nullptr;  // round start and end (timing events)
     bool busy = false;
     std::vector<size_t> finishing;
     std::vector<BWorld> h_recs;
@@ -141,6 +148,115 @@ struct Slot {  // one in-flight round
     void* d_mods = nullptr;
     void* d_pre = nullptr;
     size_t cap_recs = 0, cap_mods = 0, cap_pre = 0;
+    // final grids of verified requests, copied back with the round
+    uint32_t* p_ver = nullptr;
+    size_t cap_ver = 0;
+    std::vector<std::pair<size_t, size_t>> ver;  // (request, cell offset in p_ver)
+    int hashing = 0;     // grids of p_ver not yet hashed (guarded by the hasher's mutex)
+    double cells = 0;    // cell-updates the round runs
+    Slot() = default;
+    Slot(const Slot&) = delete;
+    Slot& operator=(const Slot&) = delete;
+    ~Slot() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev) (void)hipEventDestroy(ev);
+        if (d_recs) (void)hipFree(d_recs);
+        if (d_mods) (void)hipFree(d_mods);
+        if (d_pre) (void)hipFree(d_pre);
+        if (p_recs) (void)hipHostFree(p_recs);
+        if (p_mods) (void)hipHostFree(p_mods);
+        if (p_pre) (void)hipHostFree(p_pre);
+        if (p_ver) (void)hipHostFree(p_ver);
+    }
+};
+
+// Round time = fixed_ms + cells / cap, learned from the rounds' GPU time.
+// Rounds are classified by their cell-updates alone (not by the model, which
+// would let a wrong estimate confirm itself): rounds of under 1e8 cell-updates
+// update the fixed part, rounds of over 1e9 the capacity (exponential averages,
+// clamped).
+struct RoundModel {
+    double fixed_ms = 0.05;
+    double cap = 1.5e12;  // cell-updates per second
+    double predict_ms(double cells) const { return fixed_ms + cells / cap * 1e3; }
+    void observe(double cells, double ms) {
+        if (!(ms > 0)) return;
+        if (cells < 1e8) {
+            double f = ms - cells / cap * 1e3;
+            fixed_ms = std::min(0.5, std::max(0.01, 0.8 * fixed_ms + 0.2 * f));
+        } else if (cells > 1e9) {
+            double c = cells / (std::max(ms - fixed_ms, 0.5 * ms) * 1e-3);
+            cap = std::min(2e13, std::max(1e10, 0.7 * cap + 0.3 * c));
+        }
+    }
+};
+
+// Deadline admission plans for this share of the latency objective (the rest
+// absorbs host polling and the model's error).
+constexpr double kSloHeadroom = 0.85;
+
+// Stream synchronisation at scope exit: rounds still on the device finish
+// before the slots' buffers and the arena are freed (early error returns too).
+struct StreamDrain {
+    hipStream_t s;
+    ~StreamDrain() { (void)hipStreamSynchronize(s); }
+};
+
+uint64_t fnv64(const uint32_t* p, size_t n);
+
+// Hashes verified requests' final grids on a thread of its own, so that the
+// serving loop does not stall on them.  A slot's staging buffer is rewritten
+// only after its previous grids are hashed (wait_idle).
+struct Hasher {
+    struct Job {
+        uint64_t* hash;
+        int* verified;
+        const uint32_t* cells;
+        size_t n;
+        Slot* slot;
+    };
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Job> q;
+    bool stop = false;
+    std::thread th;
+    Hasher() : th([this] { loop(); }) {}
+    ~Hasher() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        th.join();
+    }
+    void push(const Job& j) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            j.slot->hashing++;
+            q.push_back(j);
+        }
+        cv.notify_all();
+    }
+    void wait_idle(Slot& sl) {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return sl.hashing == 0; });
+    }
+    void loop() {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            cv.wait(lk, [&] { return stop || !q.empty(); });
+            if (q.empty()) return;  // stop requested and nothing left
+            Job j = q.front();
+            q.pop_front();
+            lk.unlock();
+            uint64_t h = fnv64(j.cells, j.n);
+            lk.lock();
+            *j.hash = h;
+            *j.verified = 1;
+            j.slot->hashing--;
+            cv.notify_all();
+        }
+    }
 };
 
 int grow(void** dev, void** pinned, size_t* cap, size_t need, size_t elem) {
@@ -150,8 +266,20 @@ int grow(void** dev, void** pinned, size_t* cap, size_t need, size_t elem) {
     if (*pinned) (void)hipHostFree(*pinned);
     *dev = nullptr;
     *pinned = nullptr;
+    *cap = 0;
     HIPCHECK(hipMalloc(dev, n * elem));
     HIPCHECK(hipHostMalloc(pinned, n * elem, hipHostMallocDefault));
+    *cap = n;
+    return SE_OK;
+}
+
+int grow_pinned(uint32_t** pinned, size_t* cap, size_t cells) {
+    if (cells <= *cap) return SE_OK;
+    size_t n = std::max(cells, *cap * 2);
+    if (*pinned) (void)hipHostFree(*pinned);
+    *pinned = nullptr;
+    *cap = 0;
+    HIPCHECK(hipHostMalloc((void**)pinned, n * sizeof(uint32_t), hipHostMallocDefault));
     *cap = n;
     return SE_OK;
 }
@@ -171,6 +299,7 @@ struct se_serve {
     std::vector<FillSpec> fills;
     se_serve_config cfg{};
     std::vector<Request> reqs;
+    bool ran = false;
 };
 
 namespace {
@@ -251,13 +380,22 @@ int admit(se_serve* S, Request& r, Arena& A) {
                   (unsigned long long)r.fill_seed, fs);
 }
 
-// Steps the world takes in this round, starting at step r.scheduled.
-uint32_t quantum(const se_serve* S, const Request& r, const se::LdsVariant& v) {
-    uint32_t q = r.steps - r.scheduled;
+int lds_index(const se_module* m) { return se_rt::default_lds_index(m); }
+
+// Steps per round of a request's world before cuts at modification steps: at
+// most the tile halo for worlds spanning several LDS tiles, quantum_small for
+// worlds inside one tile (batched kernel); `quantum` for the plain kernel.
+uint32_t quantum_cap(const se_serve* S, const Request& r) {
+    if (S->cfg.kernel != 1) return std::max<uint32_t>(1, S->cfg.quantum);
+    const se_module* m = S->rs[r.rs]->mod.get();
+    const se::LdsVariant& v = m->lds[lds_index(m)].first;
     bool single = r.side <= (uint32_t)v.tw && r.side <= (uint32_t)v.th;
-    uint32_t cap = S->cfg.kernel == 1 ? (single ? S->cfg.quantum_small : std::min<uint32_t>(S->cfg.quantum, v.hb))
-                                      : S->cfg.quantum;
-    q = std::min(q, std::max<uint32_t>(1, cap));
+    return std::max<uint32_t>(1, single ? S->cfg.quantum_small : std::min<uint32_t>(S->cfg.quantum, v.hb));
+}
+
+// Steps the world takes in this round, starting at step r.scheduled.
+uint32_t quantum(const se_serve* S, const Request& r) {
+    uint32_t q = std::min(r.steps - r.scheduled, quantum_cap(S, r));
     // the next modification step after this round's first step ends the round
     for (size_t j = r.next_mod; j < r.mods.size(); j++)
         if (r.mods[j].step > r.scheduled) {
@@ -267,14 +405,56 @@ uint32_t quantum(const se_serve* S, const Request& r, const se::LdsVariant& v) {
     return q;
 }
 
-int lds_index(const se_module* m) { return se_rt::default_lds_index(m); }
+// Rounds a request still needs: its remaining steps over the quantum, plus one
+// for every later modification step (each may cut a round short).
+double rounds_left(const se_serve* S, const Request& r) {
+    uint32_t rem = r.steps - r.scheduled, q = quantum_cap(S, r);
+    double n = (double)((rem + q - 1) / q);
+    uint32_t last = r.scheduled;
+    for (size_t j = r.next_mod; j < r.mods.size(); j++)
+        if (r.mods[j].step > last) {
+            n += 1;
+            last = r.mods[j].step;
+        }
+    return n;
+}
+
+// Cell-updates the request contributes to each of its rounds.
+double round_cells(const se_serve* S, const Request& r) {
+    return (double)r.side * r.side * std::min(r.steps - r.scheduled, quantum_cap(S, r));
+}
+
+// After the round's kernels: final grids of verified requests finishing in it
+// are copied to pinned host memory on the same stream (hashed at retirement).
+int stage_verify(se_serve* S, Slot& sl, Hasher& hs, hipStream_t st) {
+    sl.ver.clear();
+    if (!S->cfg.verify_every) return SE_OK;
+    size_t need = 0;
+    for (size_t i : sl.finishing)
+        if (i % S->cfg.verify_every == 0) need += (size_t)S->reqs[i].side * S->reqs[i].side;
+    if (!need) return SE_OK;
+    hs.wait_idle(sl);  // the staging buffer still holds grids being hashed
+    int rc = grow_pinned(&sl.p_ver, &sl.cap_ver, need);
+    if (rc) return rc;
+    size_t off = 0;
+    for (size_t i : sl.finishing) {
+        if (i % S->cfg.verify_every) continue;
+        Request& r = S->reqs[i];
+        HIPCHECK(hipMemcpy2DAsync(sl.p_ver + off, (size_t)r.side * 4, r.w->buf[r.w->cur], (size_t)r.w->pitch * 4,
+                                  (size_t)r.side * 4, r.side, hipMemcpyDeviceToHost, st));
+        sl.ver.push_back({i, off});
+        off += (size_t)r.side * r.side;
+    }
+    return SE_OK;
+}
 
 // Builds and enqueues one round over every admitted world with steps left.
-int enqueue_round(se_serve* S, Slot& sl, std::vector<size_t>& active, bool* any) {
+int enqueue_round(se_serve* S, Slot& sl, Hasher& hs, std::vector<size_t>& active, bool* any) {
     *any = false;
     sl.finishing.clear();
     sl.h_recs.clear();
     sl.h_mods.clear();
+    sl.cells = 0;
     hipStream_t st = S->ctx->stream;
     // group by module (rule set)
     std::map<se_module*, std::vector<size_t>> groups;
@@ -284,21 +464,24 @@ int enqueue_round(se_serve* S, Slot& sl, std::vector<size_t>& active, bool* any)
     }
     if (groups.empty()) return SE_OK;
     *any = true;
+    HIPCHECK(hipEventRecord(sl.ev0, st));
     if (S->cfg.kernel == 0) {  // plain kernel, world by world
         for (auto& g : groups)
             for (size_t i : g.second) {
                 Request& r = S->reqs[i];
-                const se::LdsVariant& v = r.w->mod->lds[lds_index(r.w->mod.get())].first;
-                uint32_t q = quantum(S, r, v);
+                uint32_t q = quantum(S, r);
                 while (r.next_mod < r.mods.size() && r.mods[r.next_mod].step == r.scheduled)
                     r.w->pending.push_back(r.mods[r.next_mod++].m);
                 for (uint32_t s = 0; s < q; s++) {
                     int rc = se_rt::plain_step(r.w);
                     if (rc) return rc;
                 }
+                sl.cells += (double)r.side * r.side * q;
                 r.scheduled += q;
                 if (r.scheduled == r.steps) sl.finishing.push_back(i);
             }
+        int rc = stage_verify(S, sl, hs, st);
+        if (rc) return rc;
         HIPCHECK(hipEventRecord(sl.ev, st));
         sl.busy = true;
         return SE_OK;
@@ -312,7 +495,7 @@ int enqueue_round(se_serve* S, Slot& sl, std::vector<size_t>& active, bool* any)
         size_t b0 = sl.h_recs.size();
         for (size_t i : g.second) {
             Request& r = S->reqs[i];
-            uint32_t q = quantum(S, r, v);
+            uint32_t q = quantum(S, r);
             BWorld b{};
             b.buf[0] = r.w->buf[0];
             b.buf[1] = r.w->buf[1];
@@ -333,6 +516,7 @@ int enqueue_round(se_serve* S, Slot& sl, std::vector<size_t>& active, bool* any)
             r.w->frame += (int32_t)q;
             r.w->cur ^= 1;
             r.scheduled += q;
+            sl.cells += (double)r.side * r.side * q;
             if (r.scheduled == r.steps) sl.finishing.push_back(i);
         }
         ranges.push_back({mod, {b0, sl.h_recs.size()}});
@@ -375,25 +559,28 @@ int enqueue_round(se_serve* S, Slot& sl, std::vector<size_t>& active, bool* any)
                        (int)(1 << 30), (const uint32_t*)mod->table);
         if (rc) return rc;
     }
+    rc = stage_verify(S, sl, hs, st);
+    if (rc) return rc;
     HIPCHECK(hipEventRecord(sl.ev, st));
     sl.busy = true;
     return SE_OK;
 }
 
-int retire(se_serve* S, Slot& sl, Arena& A, std::chrono::steady_clock::time_point t0, uint64_t* done,
-           double* cells_done) {
+int retire(se_serve* S, Slot& sl, Hasher& hs, Arena& A, RoundModel& M, std::chrono::steady_clock::time_point t0,
+           uint64_t* done, double* cells_done) {
     double t = now_s(t0);
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, sl.ev0, sl.ev));
+    M.observe(sl.cells, ms);
+    for (auto& v : sl.ver) {
+        Request& r = S->reqs[v.first];
+        hs.push({&r.hash, &r.verified, sl.p_ver + v.second, (size_t)r.side * r.side, &sl});
+    }
+    sl.ver.clear();
     for (size_t i : sl.finishing) {
         Request& r = S->reqs[i];
         r.done_t = t;
         r.state = 2;
-        if (S->cfg.verify_every && i % S->cfg.verify_every == 0) {
-            std::vector<uint32_t> g((size_t)r.side * r.side);
-            HIPCHECK(hipMemcpy2D(g.data(), (size_t)r.side * 4, r.w->buf[r.w->cur], (size_t)r.w->pitch * 4,
-                                 (size_t)r.side * 4, r.side, hipMemcpyDeviceToHost));
-            r.hash = fnv64(g.data(), g.size());
-            r.verified = 1;
-        }
         *cells_done += (double)r.side * r.side * r.steps;
         delete r.w;  // buffers belong to the arena
         r.w = nullptr;
@@ -405,6 +592,11 @@ int retire(se_serve* S, Slot& sl, Arena& A, std::chrono::steady_clock::time_poin
     return SE_OK;
 }
 
+size_t request_bytes(const Request& r) {
+    uint32_t pitch = (r.side + 63u) & ~63u;
+    return 2 * (size_t)pitch * r.side * 4 + sizeof(se_modification) * SE_MAX_MODIFICATIONS;
+}
+
 }  // namespace
 
 extern "C" {
@@ -414,7 +606,8 @@ int se_serve_create(se_ctx* ctx, const se_ruleset* const* rulesets, int n_rulese
     if (!ctx || !rulesets || n_rulesets < 1 || n_rulesets > 16 || !cfg || !out) return fail(SE_ERR_ARG, "bad arguments");
     if (!(cfg->rate > 0) || !(cfg->duration_s > 0) || cfg->side_min < 2 || cfg->side_max < cfg->side_min ||
         cfg->side_max > 16384 || cfg->steps_min < 1 || cfg->steps_max < cfg->steps_min || cfg->mods_max > 256 ||
-        cfg->hbm_budget < (1u << 20) || (cfg->kernel != 0 && cfg->kernel != 1))
+        cfg->steps_max > (uint32_t)INT32_MAX - 1 || cfg->hbm_budget < (1u << 20) ||
+        (cfg->kernel != 0 && cfg->kernel != 1) || !(cfg->slo_ms >= 0) || !(cfg->capacity_gcups >= 0))
         return fail(SE_ERR_ARG, "bad serving configuration");
     auto* S = new se_serve();
     S->ctx = ctx;
@@ -492,40 +685,99 @@ int se_serve_request_mod(const se_serve* S, uint64_t i, uint32_t j, uint32_t* st
 
 int se_serve_run(se_serve* S, se_serve_stats* stats) {
     if (!S || !stats) return fail(SE_ERR_ARG, "null argument");
+    if (S->ran) return fail(SE_ERR_STATE, "a serving run has already taken place on this object");
+    S->ran = true;
     DeviceGuard dg(S->ctx->device);
+    const double slo = S->cfg.slo_ms;
     Arena A;
     int rc = A.init(S->cfg.hbm_budget);
     if (rc) return rc;
     std::vector<Slot> slots(S->cfg.max_inflight);
-    for (auto& sl : slots) HIPCHECK(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+    StreamDrain drain{S->ctx->stream};  // destroyed before the slots and the arena
+    Hasher hasher;                      // joined before the stream drains and the slots go
+    for (auto& sl : slots) {
+        HIPCHECK(hipEventCreate(&sl.ev0));
+        HIPCHECK(hipEventCreate(&sl.ev));
+    }
+    RoundModel M;
+    if (S->cfg.capacity_gcups > 0) M.cap = S->cfg.capacity_gcups * 1e9;
     std::deque<size_t> queue;  // arrived, waiting for admission
     std::vector<size_t> active;
     std::deque<int> order;  // busy slots in submission order
     size_t next = 0;
-    uint64_t done = 0, rounds = 0, rejected = 0;
+    uint64_t done = 0, rounds = 0, rejected = 0, shed = 0;
     double cells_done = 0;
     HIPCHECK(hipStreamSynchronize(S->ctx->stream));
     auto t0 = std::chrono::steady_clock::now();
     for (;;) {
         double t = now_s(t0);
         while (next < S->reqs.size() && S->reqs[next].arrival <= t) queue.push_back(next++);
-        // admission: FIFO while the arena has room; a request larger than the whole
-        // budget is rejected
-        while (!queue.empty()) {
-            Request& r = S->reqs[queue.front()];
-            uint32_t pitch = (r.side + 63u) & ~63u;
-            size_t need = 2 * (size_t)pitch * r.side * 4 + sizeof(se_modification) * SE_MAX_MODIFICATIONS;
-            if (need > A.size) {
-                r.state = 3;
-                rejected++;
+        if (slo <= 0) {
+            // admission: FIFO while the arena has room; a request larger than the
+            // whole budget is rejected
+            while (!queue.empty()) {
+                Request& r = S->reqs[queue.front()];
+                if (request_bytes(r) > A.size) {
+                    r.state = 3;
+                    rejected++;
+                    queue.pop_front();
+                    continue;
+                }
+                int a = admit(S, r, A);
+                if (a == 1) break;
+                if (a) return a;
+                active.push_back(queue.front());
                 queue.pop_front();
-                continue;
             }
-            int a = admit(S, r, A);
-            if (a == 1) break;
-            if (a) return a;
-            active.push_back(queue.front());
-            queue.pop_front();
+        } else if (!queue.empty()) {
+            // deadline admission.  Rounds already on the device run first (pend);
+            // a request with n rounds left then finishes after n rounds of
+            // T(U) = fixed + U / capacity, U the cell-updates of one round.
+            // Admitting r raises U by u_r; it must keep every admitted request i
+            // within n_i * T(U) <= deadline_i - now - pend, and r itself too.
+            double now = now_s(t0) * 1e3;
+            const double budget_ms = slo * kSloHeadroom;
+            double U = 0, pend = 0, tmax = 1e300;
+            for (int k : order) pend += M.predict_ms(slots[k].cells);
+            for (size_t i : active) {
+                const Request& r = S->reqs[i];
+                if (r.state != 1 || r.scheduled >= r.steps) continue;
+                U += round_cells(S, r);
+                tmax = std::min(tmax, (r.arrival * 1e3 + budget_ms - now - pend) / rounds_left(S, r));
+            }
+            for (auto it = queue.begin(); it != queue.end();) {
+                Request& r = S->reqs[*it];
+                if (request_bytes(r) > A.size) {
+                    r.state = 3;
+                    rejected++;
+                    it = queue.erase(it);
+                    continue;
+                }
+                double n = rounds_left(S, r), u = round_cells(S, r);
+                double slack = r.arrival * 1e3 + budget_ms - now - pend;
+                if (n * M.predict_ms(u) > slack) {  // cannot make it even alone
+                    r.state = 4;
+                    shed++;
+                    it = queue.erase(it);
+                    continue;
+                }
+                double tn = M.predict_ms(U + u);
+                if (n * tn > slack || tn > tmax) {  // would be late, or make another late: wait
+                    ++it;
+                    continue;
+                }
+                int a = admit(S, r, A);
+                if (a == 1) {  // no room in the arena now
+                    ++it;
+                    continue;
+                }
+                if (a) return a;
+                active.push_back(*it);
+                r.pred_ms = now + pend + n * tn - r.arrival * 1e3;
+                U += u;
+                tmax = std::min(tmax, slack / n);
+                it = queue.erase(it);
+            }
         }
         // retire completed rounds
         while (!order.empty()) {
@@ -533,7 +785,7 @@ int se_serve_run(se_serve* S, se_serve_stats* stats) {
             hipError_t q = hipEventQuery(sl.ev);
             if (q == hipErrorNotReady) break;
             if (q != hipSuccess) return fail(SE_ERR_HIP, std::string("round event: ") + hipGetErrorString(q));
-            rc = retire(S, sl, A, t0, &done, &cells_done);
+            rc = retire(S, sl, hasher, A, M, t0, &done, &cells_done);
             if (rc) return rc;
             order.pop_front();
         }
@@ -547,7 +799,7 @@ int se_serve_run(se_serve* S, se_serve_stats* stats) {
                 if (!slots[i].busy) { free_slot = i; break; }
             if (free_slot >= 0) {
                 bool any = false;
-                rc = enqueue_round(S, slots[free_slot], active, &any);
+                rc = enqueue_round(S, slots[free_slot], hasher, active, &any);
                 if (rc) return rc;
                 if (any) {
                     order.push_back(free_slot);
@@ -578,30 +830,43 @@ int se_serve_run(se_serve* S, se_serve_stats* stats) {
     }
     HIPCHECK(hipStreamSynchronize(S->ctx->stream));
     double wall = now_s(t0);
+    for (auto& sl : slots) hasher.wait_idle(sl);  // every verified grid hashed
     // statistics over completed requests
-    std::vector<double> lat, lat_all;
-    double offered = 0, processed = 0;
+    std::vector<double> lat, lat_all, ratio;
+    double offered = 0, processed = 0, shed_cells = 0;
+    uint64_t verified = 0, met = 0;
     for (auto& r : S->reqs) {
-        offered += (double)r.side * r.side * r.steps;
+        double work = (double)r.side * r.side * r.steps;
+        offered += work;
         processed += (double)r.side * r.side * r.scheduled;  // every enqueued round has completed here
-        if (r.state == 2) lat.push_back((r.done_t - r.arrival) * 1e3);
+        if (r.state == 4) shed_cells += work;
+        if (r.state == 2) {
+            double l = (r.done_t - r.arrival) * 1e3;
+            lat.push_back(l);
+            if (slo <= 0 || l <= slo) met++;
+            if (r.pred_ms > 0) ratio.push_back(l / r.pred_ms);
+        }
+        verified += (uint64_t)r.verified;
         // censored view: an arrived request not done by the end counts as done at the end
-        if (r.arrival <= wall && r.state != 3) lat_all.push_back(((r.state == 2 ? r.done_t : wall) - r.arrival) * 1e3);
+        if (r.arrival <= wall && r.state != 3 && r.state != 4)
+            lat_all.push_back(((r.state == 2 ? r.done_t : wall) - r.arrival) * 1e3);
     }
     std::sort(lat_all.begin(), lat_all.end());
     std::sort(lat.begin(), lat.end());
-    auto pct = [&](double p) {
-        if (lat.empty()) return 0.0;
-        size_t k = (size_t)std::min<double>(lat.size() - 1, std::floor(p * (lat.size() - 1) + 0.5));
-        return lat[k];
+    std::sort(ratio.begin(), ratio.end());
+    auto pct = [](const std::vector<double>& v, double p) {
+        if (v.empty()) return 0.0;
+        size_t k = (size_t)std::min<double>(v.size() - 1, std::floor(p * (v.size() - 1) + 0.5));
+        return v[k];
     };
     std::memset(stats, 0, sizeof *stats);
     stats->requests = S->reqs.size();
     stats->completed = done;
     stats->rejected = rejected;
-    stats->unfinished = S->reqs.size() - done - rejected;
-    stats->p50_ms = pct(0.50);
-    stats->p99_ms = pct(0.99);
+    stats->shed = shed;
+    stats->unfinished = S->reqs.size() - done - rejected - shed;
+    stats->p50_ms = pct(lat, 0.50);
+    stats->p99_ms = pct(lat, 0.99);
     stats->max_ms = lat.empty() ? 0 : lat.back();
     double sum = 0;
     for (double x : lat) sum += x;
@@ -610,25 +875,18 @@ int se_serve_run(se_serve* S, se_serve_stats* stats) {
     stats->cell_updates = cells_done;
     stats->gcups = cells_done / wall / 1e9;
     stats->processed_gcups = processed / wall / 1e9;
-    auto pct_all = [&](double p) {
-        if (lat_all.empty()) return 0.0;
-        size_t k = (size_t)std::min<double>(lat_all.size() - 1, std::floor(p * (lat_all.size() - 1) + 0.5));
-        return lat_all[k];
-    };
-    stats->p50_ms_censored = pct_all(0.50);
-    stats->p99_ms_censored = pct_all(0.99);
+    stats->p50_ms_censored = pct(lat_all, 0.50);
+    stats->p99_ms_censored = pct(lat_all, 0.99);
     stats->offered_gcups = offered / S->cfg.duration_s / 1e9;
     stats->peak_bytes = A.peak;
     stats->rounds = rounds;
-    for (auto& sl : slots) {
-        (void)hipEventDestroy(sl.ev);
-        if (sl.d_recs) (void)hipFree(sl.d_recs);
-        if (sl.d_mods) (void)hipFree(sl.d_mods);
-        if (sl.d_pre) (void)hipFree(sl.d_pre);
-        if (sl.p_recs) (void)hipHostFree(sl.p_recs);
-        if (sl.p_mods) (void)hipHostFree(sl.p_mods);
-        if (sl.p_pre) (void)hipHostFree(sl.p_pre);
-    }
+    stats->verified = verified;
+    stats->shed_cell_frac = offered > 0 ? shed_cells / offered : 0;
+    stats->slo_met_frac = lat.empty() ? 1.0 : (double)met / lat.size();
+    stats->capacity_gcups_est = M.cap / 1e9;
+    stats->round_fixed_ms = M.fixed_ms;
+    stats->pred_ratio_p50 = pct(ratio, 0.50);
+    stats->pred_ratio_p99 = pct(ratio, 0.99);
     for (auto& r : S->reqs)
         if (r.w) {
             delete r.w;
