@@ -323,6 +323,8 @@ def lib():
                                     P(ctypes.c_int32), ctypes.c_int]
         _lib.se_ref_hash43.argtypes = [ctypes.c_uint32] * 3 + [P(ctypes.c_uint32), P(ctypes.c_float)]
         _lib.se_ref_validate.argtypes = [P(ctypes.c_int32), ctypes.c_int]
+        _lib.se_ref_block_table.argtypes = [P(ctypes.c_int32), ctypes.c_int, ctypes.c_int, P(ctypes.c_uint32),
+                                            ctypes.c_int, P(ctypes.c_uint8)]
     return _lib
 
 
@@ -344,6 +346,18 @@ def hash43(x: int, y: int, f: int):
     lib().se_ref_hash43(x & 0xFFFFFFFF, y & 0xFFFFFFFF, f & 0xFFFFFFFF, _p(u, ctypes.c_uint32),
                         _p(r, ctypes.c_float))
     return u, r
+
+
+def block_table(blob: np.ndarray, nid: int, hashes: np.ndarray) -> np.ndarray:
+    """simulate() on every block of ids in [0, nid) under each row of `hashes`
+    (hash43 words, shape (nh, 4)): uint8[nid^4, nh, 4], tuple index
+    c0 + nid*c1 + nid^2*c2 + nid^3*c3."""
+    h = np.ascontiguousarray(hashes, dtype=np.uint32)
+    out = np.zeros((nid ** 4, len(h), 4), np.uint8)
+    if lib().se_ref_block_table(_p(blob, ctypes.c_int32), len(blob), nid, _p(h, ctypes.c_uint32), len(h),
+                                _p(out, ctypes.c_uint8)):
+        raise OracleError("se_ref_block_table failed")
+    return out
 
 
 def step(blob: np.ndarray, grid: np.ndarray, frame: int, mods=None, percell=False) -> np.ndarray:

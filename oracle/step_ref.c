@@ -210,16 +210,10 @@ static int get_cell(const ruleset_t *rs, const uint32_t *g, int W, int H, int x,
     return v < (uint32_t)rs->nmat ? (int)v : MAT_NULL;
 }
 
-/* simulate() for the block with origin (ox,oy): fills c[0..3] with the result. */
-static void sim_block(const ruleset_t *rs, const uint32_t *g, int W, int H, int ox, int oy,
-                      int frame, int c[4]) {
-    c[0] = get_cell(rs, g, W, H, ox, oy);
-    c[1] = get_cell(rs, g, W, H, ox + 1, oy);
-    c[2] = get_cell(rs, g, W, H, ox, oy + 1);
-    c[3] = get_cell(rs, g, W, H, ox + 1, oy + 1);
+/* simulate() after the block's cells are read (falling_sand.glsl:72-118): the
+ * all-EMPTY early-out, then mirror, rules and un-mirror with the given rand. */
+static void sim_cells(const ruleset_t *rs, int c[4], const float rnd[4]) {
     if (c[0] == MAT_EMPTY && c[1] == MAT_EMPTY && c[2] == MAT_EMPTY && c[3] == MAT_EMPTY) return;
-    float rnd[4];
-    se_ref_hash43((uint32_t)ox, (uint32_t)oy, (uint32_t)frame, NULL, rnd);
     int mirror = rnd[0] < 0.5f;
     if (mirror) { gswap(rs, &c[0], &c[1]); gswap(rs, &c[2], &c[3]); }
     for (int r = 0; r < rs->nrules; r++)
@@ -230,6 +224,19 @@ static void sim_block(const ruleset_t *rs, const uint32_t *g, int W, int H, int 
         if ((!mirror && k == 2) || (mirror && k == 1)) run_rule(rs, &rs->rules[r], c, rnd);
     }
 }
+
+/* simulate() for the block with origin (ox,oy): fills c[0..3] with the result. */
+static void sim_block(const ruleset_t *rs, const uint32_t *g, int W, int H, int ox, int oy,
+                      int frame, int c[4]) {
+    c[0] = get_cell(rs, g, W, H, ox, oy);
+    c[1] = get_cell(rs, g, W, H, ox + 1, oy);
+    c[2] = get_cell(rs, g, W, H, ox, oy + 1);
+    c[3] = get_cell(rs, g, W, H, ox + 1, oy + 1);
+    float rnd[4];
+    se_ref_hash43((uint32_t)ox, (uint32_t)oy, (uint32_t)frame, NULL, rnd);
+    sim_cells(rs, c, rnd);
+}
+
 
 /* falling_sand.glsl:134-179: returns the override material or -1. */
 static int mod_override(const ruleset_t *rs, const int32_t *mods, int nmods, int x, int y) {
@@ -336,5 +343,26 @@ int se_ref_run(const int32_t *blob, int len, uint32_t *grid, int W, int H, int f
     }
     if (a != grid) memcpy(grid, a, sizeof(uint32_t) * (size_t)W * H);
     free(tmp);
+    return 0;
+}
+
+/* Every block of ids in [0, nid) under each of `nh` hash43 outputs (4 words
+ * each, rand = (float)h / 2^32 as in math.glsl:74-79): out[((t * nh) + k) * 4 + i]
+ * is cell i of the result for tuple t = c0 + nid*c1 + nid^2*c2 + nid^3*c3 under
+ * hash k.  For exhaustive comparison with other block evaluators. */
+int se_ref_block_table(const int32_t *blob, int len, int nid, const uint32_t *hashes, int nh, uint8_t *out) {
+    const ruleset_t *rs = get_rs(blob, len);
+    if (!rs || nid < 1 || nid > rs->nmat || nid > 256 || nh < 1) return -1;
+    const long nt = (long)nid * nid * nid * nid;
+    #pragma omp parallel for schedule(static)
+    for (long t = 0; t < nt; t++) {
+        for (int k = 0; k < nh; k++) {
+            int c[4] = {(int)(t % nid), (int)(t / nid % nid), (int)(t / nid / nid % nid), (int)(t / nid / nid / nid)};
+            float rnd[4];
+            for (int l = 0; l < 4; l++) rnd[l] = (float)hashes[4 * k + l] / 4294967296.0f;
+            sim_cells(rs, c, rnd);
+            for (int i = 0; i < 4; i++) out[(t * nh + k) * 4 + i] = (uint8_t)c[i];
+        }
+    }
     return 0;
 }
