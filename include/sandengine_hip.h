@@ -222,7 +222,7 @@ int se_batch_tuned(const se_batch* b, const se_ruleset* rs, int32_t* variant, ui
  * (continuous batching) and each result equals the world stepped alone.
  * With slo_ms > 0 admission is also by deadline: the loop predicts each
  * request's completion from the rounds it needs and the round time (a fixed
- * part plus the round's cell-updates over a capacity learned online from the
+ * part plus the round's work over a capacity, both fitted online to the
  * rounds' GPU time); a queued request is admitted only when it and every
  * admitted request are predicted to finish within arrival + 0.85 slo_ms (the
  * rest is headroom for host delays and model error), waits while it could
@@ -242,7 +242,9 @@ typedef struct {
     size_t hbm_budget;        /* bytes of world state admitted at once */
     uint32_t verify_every;    /* keep a hash of every n-th request's final grid (0 = none) */
     double slo_ms;            /* deadline admission (see above); 0 = FIFO by HBM budget only */
-    double capacity_gcups;    /* starting value of the online capacity estimate; 0 = 1500 */
+    double capacity_gcups;    /* starting value of the online capacity estimate (G per second of the
+                                 round model's work unit: cell-updates of whole LDS tiles for the batched
+                                 kernel, cell-updates for the plain one); 0 = 1500 */
 } se_serve_config;
 
 typedef struct {
@@ -256,7 +258,7 @@ typedef struct {
     uint64_t shed, verified;                /* requests shed by deadline admission; final grids hashed */
     double shed_cell_frac;                  /* shed requests' share of the offered cell-updates */
     double slo_met_frac;                    /* completed requests done within slo_ms (1 when slo_ms is 0) */
-    double capacity_gcups_est;              /* the capacity estimate at the end of the run */
+    double capacity_gcups_est;              /* the capacity estimate at the end of the run (same unit) */
     double round_fixed_ms;                  /* the fixed part of the round-time model */
     double pred_ratio_p50, pred_ratio_p99;  /* completed requests' latency over the latency predicted at
                                                admission (deadline admission; 0 otherwise) */

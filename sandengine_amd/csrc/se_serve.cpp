@@ -152,14 +152,17 @@ struct Slot {  // one in-flight round
     uint32_t* p_ver = nullptr;
     size_t cap_ver = 0;
     std::vector<std::pair<size_t, size_t>> ver;  // (request, cell offset in p_ver)
+    hipEvent_t ev_ver = nullptr;  // after the copies into p_ver (on the copy stream)
+    bool has_ver = false;
     int hashing = 0;     // grids of p_ver not yet hashed (guarded by the hasher's mutex)
-    double cells = 0;    // cell-updates the round runs
+    double cells = 0;    // the round's work in the round model's unit (world_work)
     Slot() = default;
     Slot(const Slot&) = delete;
     Slot& operator=(const Slot&) = delete;
     ~Slot() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev) (void)hipEventDestroy(ev);
+        if (ev_ver) (void)hipEventDestroy(ev_ver);
         if (d_recs) (void)hipFree(d_recs);
         if (d_mods) (void)hipFree(d_mods);
         if (d_pre) (void)hipFree(d_pre);
@@ -170,24 +173,35 @@ struct Slot {  // one in-flight round
     }
 };
 
-// Round time = fixed_ms + cells / cap, learned from the rounds' GPU time.
-// Rounds are classified by their cell-updates alone (not by the model, which
-// would let a wrong estimate confirm itself): rounds of under 1e8 cell-updates
-// update the fixed part, rounds of over 1e9 the capacity (exponential averages,
-// clamped).
+// Round time = fixed_ms + cells / cap, fitted to the rounds' GPU time by least
+// squares with exponential forgetting (each round's weight decays by 0.98 per
+// later round).  Until the rounds' sizes spread enough to separate the two
+// terms, and whenever the fit leaves its plausible range, the fixed part stays
+// and the capacity follows the latest rounds alone.
 struct RoundModel {
     double fixed_ms = 0.05;
     double cap = 1.5e12;  // cell-updates per second
+    double s0 = 0, s1 = 0, s2 = 0, sy = 0, sxy = 0;  // weighted sums over (x = Gcell, y = ms)
     double predict_ms(double cells) const { return fixed_ms + cells / cap * 1e3; }
     void observe(double cells, double ms) {
-        if (!(ms > 0)) return;
-        if (cells < 1e8) {
-            double f = ms - cells / cap * 1e3;
-            fixed_ms = std::min(0.5, std::max(0.01, 0.8 * fixed_ms + 0.2 * f));
-        } else if (cells > 1e9) {
-            double c = cells / (std::max(ms - fixed_ms, 0.5 * ms) * 1e-3);
-            cap = std::min(2e13, std::max(1e10, 0.7 * cap + 0.3 * c));
+        if (!(ms > 0) || !(cells >= 0)) return;
+        const double x = cells * 1e-9, lam = 0.98;
+        s0 = lam * s0 + 1;
+        s1 = lam * s1 + x;
+        s2 = lam * s2 + x * x;
+        sy = lam * sy + ms;
+        sxy = lam * sxy + x * ms;
+        const double det = s0 * s2 - s1 * s1;
+        if (s0 > 8 && det > 0.05 * s0 * s2) {
+            const double b = (s0 * sxy - s1 * sy) / det, a = (sy - b * s1) / s0;  // ms per Gcell, ms
+            if (b > 0.05 && b < 100 && a > 0.005 && a < 1.0) {
+                fixed_ms = a;
+                cap = 1e12 / b;
+                return;
+            }
         }
+        if (ms > 2 * fixed_ms && cells > 0)
+            cap = std::min(2e13, std::max(1e10, 0.7 * cap + 0.3 * cells / ((ms - fixed_ms) * 1e-3)));
     }
 };
 
@@ -200,6 +214,18 @@ constexpr double kSloHeadroom = 0.85;
 struct StreamDrain {
     hipStream_t s;
     ~StreamDrain() { (void)hipStreamSynchronize(s); }
+};
+
+// The stream that copies verified grids back, beside the rounds (default
+// priority); drained and destroyed at scope exit.
+struct CopyStream {
+    hipStream_t s = nullptr;
+    ~CopyStream() {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
 };
 
 uint64_t fnv64(const uint32_t* p, size_t n);
@@ -419,15 +445,30 @@ double rounds_left(const se_serve* S, const Request& r) {
     return n;
 }
 
-// Cell-updates the request contributes to each of its rounds.
-double round_cells(const se_serve* S, const Request& r) {
-    return (double)r.side * r.side * std::min(r.steps - r.scheduled, quantum_cap(S, r));
+// Work a world adds to a round of q steps, in the round model's unit: the
+// batched kernel's cost follows its LDS tiles (a partly filled edge tile costs
+// a full one), so the cells of its tiles times q; the plain kernel's follows
+// the cells.
+double world_work(const se_serve* S, const Request& r, uint32_t q) {
+    if (S->cfg.kernel != 1) return (double)r.side * r.side * q;
+    const se_module* m = S->rs[r.rs]->mod.get();
+    const se::LdsVariant& v = m->lds[lds_index(m)].first;
+    const double tx = (double)((r.side + v.tw - 1) / v.tw), ty = (double)((r.side + v.th - 1) / v.th);
+    return tx * ty * v.tw * v.th * q;
 }
 
-// After the round's kernels: final grids of verified requests finishing in it
-// are copied to pinned host memory on the same stream (hashed at retirement).
-int stage_verify(se_serve* S, Slot& sl, Hasher& hs, hipStream_t st) {
+// The request's work in each of its rounds.
+double round_cells(const se_serve* S, const Request& r) {
+    return world_work(S, r, std::min(r.steps - r.scheduled, quantum_cap(S, r)));
+}
+
+// After the round: final grids of verified requests finishing in it are copied
+// to pinned host memory on the copy stream, which waits for the round's end
+// event; the next rounds do not wait for the copies.  The requests' arena space
+// is released once the copies are done too (hashed at retirement).
+int stage_verify(se_serve* S, Slot& sl, Hasher& hs, hipStream_t vst) {
     sl.ver.clear();
+    sl.has_ver = false;
     if (!S->cfg.verify_every) return SE_OK;
     size_t need = 0;
     for (size_t i : sl.finishing)
@@ -436,20 +477,23 @@ int stage_verify(se_serve* S, Slot& sl, Hasher& hs, hipStream_t st) {
     hs.wait_idle(sl);  // the staging buffer still holds grids being hashed
     int rc = grow_pinned(&sl.p_ver, &sl.cap_ver, need);
     if (rc) return rc;
+    HIPCHECK(hipStreamWaitEvent(vst, sl.ev, 0));
     size_t off = 0;
     for (size_t i : sl.finishing) {
         if (i % S->cfg.verify_every) continue;
         Request& r = S->reqs[i];
         HIPCHECK(hipMemcpy2DAsync(sl.p_ver + off, (size_t)r.side * 4, r.w->buf[r.w->cur], (size_t)r.w->pitch * 4,
-                                  (size_t)r.side * 4, r.side, hipMemcpyDeviceToHost, st));
+                                  (size_t)r.side * 4, r.side, hipMemcpyDeviceToHost, vst));
         sl.ver.push_back({i, off});
         off += (size_t)r.side * r.side;
     }
+    HIPCHECK(hipEventRecord(sl.ev_ver, vst));
+    sl.has_ver = true;
     return SE_OK;
 }
 
 // Builds and enqueues one round over every admitted world with steps left.
-int enqueue_round(se_serve* S, Slot& sl, Hasher& hs, std::vector<size_t>& active, bool* any) {
+int enqueue_round(se_serve* S, Slot& sl, Hasher& hs, hipStream_t vst, std::vector<size_t>& active, bool* any) {
     *any = false;
     sl.finishing.clear();
     sl.h_recs.clear();
@@ -476,13 +520,13 @@ int enqueue_round(se_serve* S, Slot& sl, Hasher& hs, std::vector<size_t>& active
                     int rc = se_rt::plain_step(r.w);
                     if (rc) return rc;
                 }
-                sl.cells += (double)r.side * r.side * q;
+                sl.cells += world_work(S, r, q);
                 r.scheduled += q;
                 if (r.scheduled == r.steps) sl.finishing.push_back(i);
             }
-        int rc = stage_verify(S, sl, hs, st);
-        if (rc) return rc;
         HIPCHECK(hipEventRecord(sl.ev, st));
+        int rc = stage_verify(S, sl, hs, vst);
+        if (rc) return rc;
         sl.busy = true;
         return SE_OK;
     }
@@ -516,7 +560,7 @@ int enqueue_round(se_serve* S, Slot& sl, Hasher& hs, std::vector<size_t>& active
             r.w->frame += (int32_t)q;
             r.w->cur ^= 1;
             r.scheduled += q;
-            sl.cells += (double)r.side * r.side * q;
+            sl.cells += world_work(S, r, q);
             if (r.scheduled == r.steps) sl.finishing.push_back(i);
         }
         ranges.push_back({mod, {b0, sl.h_recs.size()}});
@@ -559,9 +603,9 @@ int enqueue_round(se_serve* S, Slot& sl, Hasher& hs, std::vector<size_t>& active
                        (int)(1 << 30), (const uint32_t*)mod->table);
         if (rc) return rc;
     }
-    rc = stage_verify(S, sl, hs, st);
-    if (rc) return rc;
     HIPCHECK(hipEventRecord(sl.ev, st));
+    rc = stage_verify(S, sl, hs, vst);
+    if (rc) return rc;
     sl.busy = true;
     return SE_OK;
 }
@@ -694,10 +738,13 @@ int se_serve_run(se_serve* S, se_serve_stats* stats) {
     if (rc) return rc;
     std::vector<Slot> slots(S->cfg.max_inflight);
     StreamDrain drain{S->ctx->stream};  // destroyed before the slots and the arena
-    Hasher hasher;                      // joined before the stream drains and the slots go
+    CopyStream vs;                      // drained before the slots' pinned buffers go
+    Hasher hasher;                      // joined before the streams drain and the slots go
+    HIPCHECK(hipStreamCreateWithFlags(&vs.s, hipStreamNonBlocking));
     for (auto& sl : slots) {
         HIPCHECK(hipEventCreate(&sl.ev0));
         HIPCHECK(hipEventCreate(&sl.ev));
+        HIPCHECK(hipEventCreateWithFlags(&sl.ev_ver, hipEventDisableTiming));
     }
     RoundModel M;
     if (S->cfg.capacity_gcups > 0) M.cap = S->cfg.capacity_gcups * 1e9;
@@ -783,6 +830,7 @@ int se_serve_run(se_serve* S, se_serve_stats* stats) {
         while (!order.empty()) {
             Slot& sl = slots[order.front()];
             hipError_t q = hipEventQuery(sl.ev);
+            if (q == hipSuccess && sl.has_ver) q = hipEventQuery(sl.ev_ver);
             if (q == hipErrorNotReady) break;
             if (q != hipSuccess) return fail(SE_ERR_HIP, std::string("round event: ") + hipGetErrorString(q));
             rc = retire(S, sl, hasher, A, M, t0, &done, &cells_done);
@@ -799,7 +847,7 @@ int se_serve_run(se_serve* S, se_serve_stats* stats) {
                 if (!slots[i].busy) { free_slot = i; break; }
             if (free_slot >= 0) {
                 bool any = false;
-                rc = enqueue_round(S, slots[free_slot], hasher, active, &any);
+                rc = enqueue_round(S, slots[free_slot], hasher, vs.s, active, &any);
                 if (rc) return rc;
                 if (any) {
                     order.push_back(free_slot);
@@ -822,13 +870,15 @@ int se_serve_run(se_serve* S, se_serve_stats* stats) {
             // otherwise poll briefly so that an arrival can start the next round.
             double wait = next < S->reqs.size() ? S->reqs[next].arrival - now_s(t0) : 1e9;
             if ((int)order.size() >= S->cfg.max_inflight) {
-                HIPCHECK(hipEventSynchronize(slots[order.front()].ev));
+                const Slot& sl = slots[order.front()];
+                HIPCHECK(hipEventSynchronize(sl.has_ver ? sl.ev_ver : sl.ev));
             } else if (wait > 0) {
                 std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.0002)));
             }
         }
     }
     HIPCHECK(hipStreamSynchronize(S->ctx->stream));
+    HIPCHECK(hipStreamSynchronize(vs.s));
     double wall = now_s(t0);
     for (auto& sl : slots) hasher.wait_idle(sl);  // every verified grid hashed
     // statistics over completed requests

@@ -11,7 +11,7 @@ the input SHA-256 is recorded beside them):
   glsl_sha256.json  SHA-256 of the reference's checked-in generated GLSL for
                its data/materials.yaml (pins the GLSL emitters without the tree)
 Every grid here was produced by llvmpipe; the oracle is compared against them
-in tests/test_oracle_golden.py.
+in tests/test_oracle.py, the GPU kernels in tests/test_gpu_parity.py.
 """
 from __future__ import annotations
 

@@ -1,8 +1,8 @@
 """Render side-outputs (colour noise and light) against the reference's GLSL
 under llvmpipe (tests/golden/render.npz, from tests/golden/make_render_golden.py).
 
-The grid must match exactly.  Light is f32 arithmetic in the shader's order;
-it is compared within 1e-6 (it matches exactly here).  EMPTY cells carry no
+The grid must match exactly.  Light is f32 arithmetic in the shader's order
+and must match exactly too, bit for bit.  EMPTY cells carry no
 noise and must match exactly.  The noise of the other cells goes through
 fract(sin(x) * 43758.5453): a one-ulp difference between two sin
 implementations moves a value sitting next to a wrap of fract across it, so a
@@ -59,7 +59,7 @@ def test_render_outputs_match_llvmpipe(case):
     np.testing.assert_array_equal(grid, want[case["name"] + "/grid"])
     wl, wc = want[case["name"] + "/light"], want[case["name"] + "/color"]
     print(f"  light max |d| {np.abs(light - wl).max():.3g}, colour max |d| {np.abs(col - wc).max():.3g}", flush=True)
-    np.testing.assert_allclose(light, wl, rtol=0, atol=1e-6)
+    np.testing.assert_array_equal(light, wl)  # f32 arithmetic in the shader's order: bit-exact
     empty = grid == 0
     np.testing.assert_array_equal(col[empty], wc[empty])
     d = np.abs(col - wc).max(axis=2)[~empty]
