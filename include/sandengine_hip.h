@@ -286,6 +286,39 @@ int se_serve_request_mod(const se_serve* s, uint64_t i, uint32_t j, uint32_t* st
 /* FNV-1a 64 of a byte range (the hash se_serve_request.hash uses). */
 uint64_t se_fnv1a64(const void* data, size_t bytes);
 
+/* ---------------------------------------------------------------- batched step without allocation
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, serving loops that carve worlds from
+ * their own arena, harnesses that forbid hipMalloc: the batched LDS step of
+ * se_batch_run (same kernels, same results) over grids the caller owns.  The
+ * per-call device data (launch records, tile prefix sums, modifications) is
+ * staged in a fixed array of the library, SE_NOALLOC_SCRATCH_BYTES per device,
+ * located when a context is created on the device, through a pinned host
+ * buffer of the context; the call allocates nothing.  Limits: at most
+ * SE_NOALLOC_MAX_GRIDS grids and SE_NOALLOC_MAX_MODS modifications per call
+ * (SE_ERR_ARG beyond), one rule set per call.  Every call on a device shares
+ * its array and overwrites it, so calls that overlap on one device (other
+ * threads, or other streams before the previous call's work has run) race;
+ * calls on one stream are ordered. */
+typedef struct {
+    uint32_t* grid[2];  /* device buffers of pitch*h u32 cells, row-major; the caller owns them */
+    uint32_t w, h, pitch;
+    int32_t cur;        /* in: the buffer holding the current grid; out: the one holding the result */
+    int32_t frame;      /* in: Params.frame before the steps; out: after them */
+    uint32_t nsteps;    /* steps (Simulation::run calls) this grid advances */
+    uint32_t mod_first; /* modifications for the grid's first step: mods[mod_first .. mod_first + mod_count) */
+    uint32_t mod_count; /* at most SE_MAX_MODIFICATIONS; cut at the first of size 0, as se_world_step does */
+} se_grid_step;
+#define SE_NOALLOC_MAX_GRIDS 2048
+#define SE_NOALLOC_MAX_MODS 2048
+#define SE_NOALLOC_SCRATCH_BYTES (224 * 1024)
+/* Enqueues the steps of n grids on `stream` (a hipStream_t; NULL: the
+ * context's stream) and updates grids[i].cur and .frame; returns without
+ * waiting for the device.  A grid whose next frame is the clearing frame 1
+ * takes that step first (its modifications are dropped, as there). */
+int se_grids_step_noalloc(se_ctx* ctx, const se_ruleset* rs, se_grid_step* grids, uint32_t n,
+                          const se_modification* mods, uint32_t nmods, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,13 @@ class SeModification(ctypes.Structure):
 
 assert ctypes.sizeof(SeModification) == 32
 
+
+class SeGridStep(ctypes.Structure):
+    """se_grid_step: one caller-owned grid of se_grids_step_noalloc."""
+    _fields_ = [("grid", ctypes.c_void_p * 2), ("w", ctypes.c_uint32), ("h", ctypes.c_uint32),
+                ("pitch", ctypes.c_uint32), ("cur", ctypes.c_int32), ("frame", ctypes.c_int32),
+                ("nsteps", ctypes.c_uint32), ("mod_first", ctypes.c_uint32), ("mod_count", ctypes.c_uint32)]
+
 _lib = None
 
 
@@ -133,6 +140,8 @@ def lib():
         L.se_batch_autotune.argtypes = [c_p, c_u32, ctypes.POINTER(c_i32), ctypes.POINTER(c_u32),
                                         ctypes.POINTER(c_i32), ctypes.POINTER(ctypes.c_float)]
         L.se_variant_info.argtypes = [c_p, c_i32, c_i32] + [ctypes.POINTER(c_i32)] * 3
+        L.se_grids_step_noalloc.argtypes = [c_p, c_p, ctypes.POINTER(SeGridStep), c_u32,
+                                            ctypes.POINTER(SeModification), c_u32, c_p]
         _lib = L
     return _lib
 
@@ -536,6 +545,28 @@ class Serve:
             self.close()
         except Exception:
             pass
+
+
+def grids_step_noalloc(ctx: "Context", rules: "RuleSet", grids: Sequence[dict], mods=(), stream: int = 0):
+    """se_grids_step_noalloc: the batched step over caller-owned device grids,
+    allocating nothing.  Each grid is a dict with keys grid0, grid1 (device
+    addresses of pitch*h u32 cells), w, h, pitch, cur, frame, nsteps and
+    optionally mod_first, mod_count (into `mods`: (x, y, shape, size, matID)
+    tuples).  `stream` is a hipStream_t handle (0: the context's stream).  The
+    work is enqueued, not waited for; returns the (cur, frame) of each grid after
+    it."""
+    arr = (SeGridStep * max(1, len(grids)))()
+    for i, g in enumerate(grids):
+        a = arr[i]
+        a.grid[0], a.grid[1] = g["grid0"], g["grid1"]
+        a.w, a.h, a.pitch, a.cur, a.frame, a.nsteps = g["w"], g["h"], g["pitch"], g["cur"], g["frame"], g["nsteps"]
+        a.mod_first, a.mod_count = g.get("mod_first", 0), g.get("mod_count", 0)
+    m = (SeModification * max(1, len(mods)))()
+    for i, md in enumerate(mods):
+        m[i].position[0], m[i].position[1] = md[0], md[1]
+        m[i].mod_shape, m[i].mod_size, m[i].mod_matID = md[2], md[3], md[4]
+    _check(lib().se_grids_step_noalloc(ctx._p, rules._p, arr, len(grids), m, len(mods), stream or None))
+    return [(arr[i].cur, arr[i].frame) for i in range(len(grids))]
 
 
 class Batch:

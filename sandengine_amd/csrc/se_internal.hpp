@@ -93,6 +93,12 @@ struct se_ctx {
     hipStream_t stream = nullptr;
     std::mutex mu;
     std::map<uint64_t, std::pair<std::string, std::shared_ptr<se_module>>> cache;
+    // se_grids_step_noalloc: the device's scratch array, a pinned staging buffer
+    // of its size, and the event after the last copy out of that buffer
+    void* scratch = nullptr;
+    void* staging = nullptr;
+    hipEvent_t staging_done = nullptr;
+    std::mutex staging_mu;
 };
 
 struct se_ruleset {
@@ -130,4 +136,6 @@ int plain_step(se_world* w);
 // Index of the default LDS tile variant of a module (batches and serving use it
 // unless tuned).
 int default_lds_index(const se_module* m);
+// Device address of the no-allocation scratch array on `device` (current).
+int noalloc_scratch(int device, void** out);
 }  // namespace se_rt

@@ -610,6 +610,18 @@ int se_ctx_create(int device, se_ctx** out) {
         delete c;
         return fail(SE_ERR_HIP, "hipStreamCreate failed");
     }
+    // set-up of se_grids_step_noalloc: the device's scratch array and its pinned
+    // staging.  Without them that entry point reports SE_ERR_STATE; the rest of
+    // the library does not need them.
+    if (se_rt::noalloc_scratch(device, &c->scratch) == SE_OK) {
+        if (hipHostMalloc(&c->staging, SE_NOALLOC_SCRATCH_BYTES, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&c->staging_done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            if (c->staging) (void)hipHostFree(c->staging);
+            c->staging = nullptr;
+            c->scratch = nullptr;
+        }
+    }
     *out = c;
     return SE_OK;
 }
@@ -620,6 +632,11 @@ void se_ctx_destroy(se_ctx* ctx) {
         DeviceGuard dg(ctx->device);
         hipStreamSynchronize(ctx->stream);
         hipStreamDestroy(ctx->stream);
+        if (ctx->staging_done) {
+            hipEventSynchronize(ctx->staging_done);
+            hipEventDestroy(ctx->staging_done);
+        }
+        if (ctx->staging) hipHostFree(ctx->staging);
     }
     delete ctx;
 }

@@ -20,6 +20,7 @@ STRUCTS = {
     "se_serve_config": S.ServeConfig,
     "se_serve_stats": S.ServeStats,
     "se_serve_request": S.ServeRequest,
+    "se_grid_step": S.SeGridStep,
 }
 
 
