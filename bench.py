@@ -190,7 +190,7 @@ def run_batch(args):
     fast, slow = [], []
     for side, rs, steps, fseed in specs:
         ids, pct = BATCH_FILLS[rs]
-        for lst in (fast, slow):
+        for lst in ((fast,) if args.no_looped else (fast, slow)):
             w = S.World(ctx, rsets[rs], side, side)
             w.frame = 1
             w.fill_seeded(fseed, ids, pct)
@@ -199,8 +199,9 @@ def run_batch(args):
     log(f"{len(specs)} worlds created and filled in {time.perf_counter() - t0:.1f}s; "
         f"{cells:.3e} cell-updates")
     bf, bs = S.Batch(ctx), S.Batch(ctx)
-    for (side, rs, steps, _), wf, ws in zip(specs, fast, slow):
+    for (side, rs, steps, _), wf in zip(specs, fast):
         bf.add(wf, steps)
+    for (side, rs, steps, _), ws in zip(specs, slow):
         bs.add(ws, steps)
     t0 = time.perf_counter()
     bf.autotune(rounds=args.tune_rounds)
@@ -218,13 +219,15 @@ def run_batch(args):
     t0 = time.perf_counter()
     ms_batch = bf.run(mode=1, graph=True)
     wall_batch = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    ms_loop = bs.run(mode=0)
-    wall_loop = time.perf_counter() - t0
-    log(f"batched: {ms_batch:.1f} ms GPU ({wall_batch:.2f}s wall incl. plan+graph); "
-        f"looped plain: {ms_loop:.1f} ms GPU ({wall_loop:.2f}s wall)")
+    ms_loop = wall_loop = None
+    if not args.no_looped:
+        t0 = time.perf_counter()
+        ms_loop = bs.run(mode=0)
+        wall_loop = time.perf_counter() - t0
+        log(f"batched: {ms_batch:.1f} ms GPU ({wall_batch:.2f}s wall incl. plan+graph); "
+            f"looped plain: {ms_loop:.1f} ms GPU ({wall_loop:.2f}s wall)")
     identical = True
-    if not args.no_check:
+    if not args.no_check and not args.no_looped:
         for wf, ws in zip(fast, slow):
             if wf.frame != ws.frame or not np.array_equal(wf.download(), ws.download()):
                 identical = False
@@ -246,12 +249,14 @@ def run_batch(args):
                    "tuned_variant": tuned_by_rules},
         "batched": {"gpu_ms": ms_batch, "wall_s": wall_batch, "gcups_gpu": cells / (ms_batch / 1e3) / 1e9,
                     "gcups_wall": cells / wall_batch / 1e9, "autotune_s": tune_s},
-        "looped_plain": {"gpu_ms": ms_loop, "wall_s": wall_loop, "gcups_gpu": cells / (ms_loop / 1e3) / 1e9,
-                         "gcups_wall": cells / wall_loop / 1e9},
-        "speedup_batched_vs_looped_plain": wall_loop / wall_batch,
+        "looped_plain": None if args.no_looped else {
+            "gpu_ms": ms_loop, "wall_s": wall_loop, "gcups_gpu": cells / (ms_loop / 1e3) / 1e9,
+            "gcups_wall": cells / wall_loop / 1e9},
+        "speedup_batched_vs_looped_plain": None if args.no_looped else wall_loop / wall_batch,
         "cpu_baseline": cpu,
         "cell_updates": cells,
-        "parity": {"identical": identical},
+        "parity": {"identical": None if (args.no_check or args.no_looped) else identical,
+                   "check": "every world's grid after the batched run equals the looped plain kernel's"},
     }
     print(json.dumps(out))
     if not identical:
@@ -487,6 +492,7 @@ def main():
     ap.add_argument("--paths", default="plain,blocked")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-looped", action="store_true", help="batch: skip the looped plain-kernel comparison")
     ap.add_argument("--latency-samples", type=int, default=50)
     ap.add_argument("--roofline-launches", type=int, default=10)
     ap.add_argument("--copy-reps", type=int, default=20)

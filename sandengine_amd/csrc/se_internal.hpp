@@ -65,6 +65,7 @@ struct se_module {
     size_t table_entries = 0;        // 0 when the rule set runs the generated chain
     int table_form = 0;              // 0 generated chain, 1 full table, 2 coded table
     bool wall_kept = false;          // full table: no entry changes a WALL cell
+    uint32_t stable_mask = 0;        // bit c: a block of four cells of LDS code c never changes
     std::vector<unsigned> lds_bytes; // dynamic LDS per LDS variant
     std::vector<int> lds_grid;       // resident workgroups per LDS variant at full occupancy (multiple of 8)
     std::vector<int> lds_waves;      // resident wavefronts per CU per LDS variant

@@ -294,6 +294,13 @@ __device__ __forceinline__ u32 se_enc(u32 v) { return se_enc_ok(se_decode(v)); }
 // restoring of WALL outside the world, which the table already keeps.
 extern "C" __device__ u32 se_wall_kept = 0u;
 
+// Bit c set (by the host, from the built table) when a block of four cells of
+// LDS code c comes out unchanged under every hash class and mirror bit.  A tile
+// whose cells all hold such a c is a fixed point of every sub-step: each block
+// evaluated in it holds four cells of c (se_lds_body skips its sub-steps).
+// 0 (never skip) for rule sets without a table.
+extern "C" __device__ u32 se_stable_mask = 0u;
+
 extern "C" __global__ __launch_bounds__(256) void se_build_table(u32* __restrict__ tbl) {
     const u32 tuple = blockIdx.x * 256u + threadIdx.x;
     if (tuple >= SE_TBL_ENTRIES / SE_TBL_R) return;
@@ -731,6 +738,10 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     const bool interior = gx0 >= 0 && gy0 >= 0 && gx0 + LW <= W && gy0 + LH <= H;
     u32* lds32 = reinterpret_cast<u32*>(lds);
     bool fixed = false;  // this thread stored a NULL or WALL code (coded table)
+    // interior tiles: the code of the tile's first cell in all four bytes; quads
+    // that differ from it make the tile unsettled
+    const u32 ref = interior ? se_enc(in[(size_t)gy0 * pitch + gx0]) * 0x01010101u : 0u;
+    bool mixed = false;
     // whole tile inside the world, workgroup a whole number of tile rows wide:
     // each thread keeps one quad column and steps down the rows by pointer
     // increments (no per-quad index arithmetic), U loads in flight before any is
@@ -755,6 +766,7 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
             for (int u = 0; u < U; u++) {
                 const u32 q = se_pack_quad(v[u]);
                 fixed |= se_has_fixed(q);
+                mixed |= q != ref;
                 dst[(k0 + u) * DROW * LQ] = q;
             }
         }
@@ -776,6 +788,7 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                 if (i < TOTAL) {
                     const u32 q = se_pack_quad(v[u]);
                     fixed |= se_has_fixed(q);
+                    mixed |= q != ref;
                     lds32[i] = q;
                 }
             }
@@ -852,9 +865,17 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     // holds either (table results never introduce them).  Threads that stored
     // such a cell write this item's sequence number into the LDS flag word; an
     // older number means clean, so the word needs no clearing between items.
-    if (SE_TBL == 2 && fixed) *flag = seq;
+    if (SE_TBL == 2 && fixed) flag[0] = seq;
+    if (mixed) flag[1] = seq;
     __syncthreads();
-    bool tile_fixed = SE_TBL == 2 && *flag == seq;
+    bool tile_fixed = SE_TBL == 2 && flag[0] == seq;
+    // A settled tile: interior, every cell (halo included) the same code c with
+    // bit c of se_stable_mask set, and no modification near it.  Every block of
+    // every sub-step is then four cells of c, which stay as they are: the
+    // sub-steps are skipped and the tile is stored as loaded.
+    const u32 cref = ref & 0xffu;
+    const bool settled = interior && flag[1] != seq && cref < 32u && ((se_stable_mask >> cref) & 1u) != 0u &&
+                         (nmods == 0 || !se_mods_touch(mods, nmods, gx0, gx0 + LW - 1, gy0, gy0 + LH - 1));
     // edge tiles need the WALL restoring of their block loop unless the full
     // table keeps WALL cells by itself
 #if SE_TBL == 1
@@ -862,7 +883,7 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
 #else
     const bool mask_wall = !interior;
 #endif
-    for (int s = 0; s < nsub; s++) {
+    for (int s = 0; s < (settled ? 0 : nsub); s++) {
         const u32 f = (u32)(frame0 + s);
         int ox, oy;
         se_offset(f, ox, oy);
@@ -929,9 +950,9 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                 }
             }
         }
-        if (SE_TBL == 2 && use_mods && fixed) *flag = seq;  // brushes may place WALL
+        if (SE_TBL == 2 && use_mods && fixed) flag[0] = seq;  // brushes may place WALL
         __syncthreads();
-        if (SE_TBL == 2 && use_mods) tile_fixed = *flag == seq;
+        if (SE_TBL == 2 && use_mods) tile_fixed = flag[0] == seq;
     }
     // interior store: quads from LDS, 16-byte global stores where inside the world
     constexpr int TQ = TW / 4;
@@ -1029,11 +1050,11 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     __syncthreads();  // LDS is reused by the next item of a persistent workgroup
 }
 
-// The table (SE_TBL_WORDS u32 words) into LDS after the tile; the flag word
-// after the table starts at 0.
+// The table (SE_TBL_WORDS u32 words) into LDS after the tile; the two flag
+// words after the table (NULL/WALL seen, mixed tile) start at 0.
 __device__ __forceinline__ const u32* se_stage_table(const u32* __restrict__ tbl, unsigned char* lds, int lds_tile,
                                                      int nt, u32* flag) {
-    if (threadIdx.x == 0) *flag = 0u;
+    if (threadIdx.x < 2) flag[threadIdx.x] = 0u;
 #if SE_TBL
     u32* ltbl = reinterpret_cast<u32*>(lds + ((lds_tile + 15) & ~15));
     // 16-byte loads, U per thread in flight before their LDS stores
@@ -1098,8 +1119,8 @@ __device__ __forceinline__ int se_xcd_tile(int b, int n) {
 }
 
 // LDS (static, so every table offset is an immediate): tile bytes rounded to 16,
-// the table (SE_TBL_WORDS u32), a 16-byte slot holding the tile's NULL/WALL flag
-// word
+// the table (SE_TBL_WORDS u32), a 16-byte slot holding the tile's flag words
+// (NULL/WALL seen, mixed tile)
 //  se_step_lds_*  : one world, one workgroup per tile (workgroups that start as
 //                   others retire keep tile loads and tile compute overlapped;
 //                   measured faster than persistent workgroups, which move in step).

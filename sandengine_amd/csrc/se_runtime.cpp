@@ -220,8 +220,10 @@ constexpr int kCtableUnfit = 1;  // build_ctable: the rule set does not fit the 
 
 // The coded table of a rule set: builds a module without LDS kernels, runs its
 // se_build_ctable and compacts the result on the host (compact_ctable).
+// *stable: bit c set when a block of four cells of code c comes out unchanged
+// for every class and mirror bit (see uniform_stable_mask).
 int build_ctable(se_ctx* ctx, const se::ParsingResult& pr, const se::HipGenInfo& info,
-                 std::vector<uint32_t>& words) {
+                 std::vector<uint32_t>& words, uint32_t* stable) {
     std::string src;
     int rc = gen_source(pr, src, nullptr, true, 0, false);
     if (rc) return rc;
@@ -245,6 +247,14 @@ int build_ctable(se_ctx* ctx, const se::ParsingResult& pr, const se::HipGenInfo&
     (void)hipFree(d_raw);
     (void)hipModuleUnload(mod);
     if (rc) return rc;
+    *stable = 0u;
+    for (uint32_t c = 0; c < (uint32_t)D && c < 32u; c++) {
+        const size_t ti = (size_t)c * (1 + D + D * D + D * D * D);
+        const uint32_t id = (c + 3u) % (uint32_t)info.nmat;
+        bool keep = true;
+        for (size_t k = 0; k < R * 2; k++) keep = keep && raw[ti * R * 2 + k] == id * 0x01010101u;
+        if (keep) *stable |= 1u << c;
+    }
     if (!compact_ctable(raw, (int)D, (int)R, info.nmat, words)) return kCtableUnfit;
     return variants_for(words.size()).empty() ? kCtableUnfit : SE_OK;  // no tile fits beside it
 }
@@ -269,9 +279,35 @@ bool table_keeps_wall(const std::vector<uint32_t>& tbl, size_t n, size_t R) {
     return true;
 }
 
-// ctwords: the compacted coded table when the source was generated for one.
+// Bit c set when a block of four cells of id c comes out unchanged under every
+// hash class of a full transition table (entry tuple * R + r).  A tile whose
+// cells all hold such a c is a fixed point of every sub-step (se_lds_body skips
+// its sub-steps); with swap-only rules every c qualifies.
+uint32_t uniform_stable_mask(const std::vector<uint32_t>& tbl, size_t n, size_t R) {
+    uint32_t mask = 0;
+    for (size_t c = 0; c < n && c < 32; c++) {
+        const size_t tuple = c * (1 + n + n * n + n * n * n);
+        bool keep = true;
+        for (size_t r = 0; r < R; r++) keep = keep && tbl[tuple * R + r] == (uint32_t)c * 0x01010101u;
+        if (keep) mask |= 1u << c;
+    }
+    return mask;
+}
+
+// Writes a u32 module global of a loaded module.
+int set_module_u32(hipModule_t mod, const char* name, uint32_t value) {
+    void* p = nullptr;
+    size_t bytes = 0;
+    HIPCHECK(hipModuleGetGlobal(&p, &bytes, mod, name));
+    if (bytes != sizeof(uint32_t)) return fail(SE_ERR_JIT, std::string("internal: ") + name + " has the wrong size");
+    HIPCHECK(hipMemcpy(p, &value, sizeof value, hipMemcpyHostToDevice));
+    return SE_OK;
+}
+
+// ctwords: the compacted coded table when the source was generated for one;
+// ct_stable its uniform-block mask (build_ctable).
 int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
-                const std::vector<uint32_t>* ctwords, std::shared_ptr<se_module>& out) {
+                const std::vector<uint32_t>* ctwords, uint32_t ct_stable, std::shared_ptr<se_module>& out) {
     uint64_t key = fnv1a(src);
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
@@ -320,13 +356,10 @@ int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
         HIPCHECK(hipMemcpy(tbl.data(), m->table, tbl.size() * 4, hipMemcpyDeviceToHost));
         m->wall_kept = table_keeps_wall(tbl, n, (size_t)info.table_R);
         if (m->wall_kept) {
-            void* flag = nullptr;
-            size_t flag_bytes = 0;
-            HIPCHECK(hipModuleGetGlobal(&flag, &flag_bytes, m->mod, "se_wall_kept"));
-            if (flag_bytes != sizeof(uint32_t)) return fail(SE_ERR_JIT, "internal: se_wall_kept has the wrong size");
-            const uint32_t one = 1u;
-            HIPCHECK(hipMemcpy(flag, &one, sizeof one, hipMemcpyHostToDevice));
+            rc = set_module_u32(m->mod, "se_wall_kept", 1u);
+            if (rc) return rc;
         }
+        m->stable_mask = uniform_stable_mask(tbl, n, (size_t)info.table_R);
     }
     if (info.ctable) {
         if (!ctwords) return fail(SE_ERR_JIT, "internal: coded-table source without its table");
@@ -335,6 +368,11 @@ int load_module(se_ctx* ctx, const std::string& src, const se::HipGenInfo& info,
         m->table_form = 2;
         HIPCHECK(hipMalloc(&m->table, words.size() * 4));
         HIPCHECK(hipMemcpy(m->table, words.data(), words.size() * 4, hipMemcpyHostToDevice));
+        m->stable_mask = ct_stable;
+    }
+    if (m->stable_mask) {
+        rc = set_module_u32(m->mod, "se_stable_mask", m->stable_mask);
+        if (rc) return rc;
     }
     for (auto& v : variants_for(m->table_entries)) {  // the variants the source was generated with
         hipFunction_t f;
@@ -665,17 +703,18 @@ int se_ruleset_from_yaml(se_ctx* ctx, const char* yaml, size_t len, se_ruleset**
         // the coded table is enumerated by a first build; the LDS kernels are then
         // generated with its size (static LDS), or without it when it does not fit
         std::vector<uint32_t> words;
-        rc = build_ctable(ctx, rs->pr, info, words);
+        uint32_t stable = 0;
+        rc = build_ctable(ctx, rs->pr, info, words, &stable);
         if (rc == kCtableUnfit) {
             rc = gen_source(rs->pr, src, &info, false);
-            if (!rc) rc = load_module(ctx, src, info, nullptr, rs->mod);
+            if (!rc) rc = load_module(ctx, src, info, nullptr, 0u, rs->mod);
         } else if (!rc) {
             const uint64_t t1_words = ((uint64_t)info.ct_D * info.ct_D * info.ct_D * info.ct_D + 15) / 16 * 4;
             rc = gen_source(rs->pr, src, &info, true, words.size() - t1_words);
-            if (!rc) rc = load_module(ctx, src, info, &words, rs->mod);
+            if (!rc) rc = load_module(ctx, src, info, &words, stable, rs->mod);
         }
     } else {
-        rc = load_module(ctx, src, info, nullptr, rs->mod);
+        rc = load_module(ctx, src, info, nullptr, 0u, rs->mod);
     }
     if (rc) return rc;
     rs->ctx = ctx;
