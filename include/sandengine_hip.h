@@ -128,6 +128,13 @@ int se_world_step_blocked(se_world* w, uint32_t nsteps, uint32_t sub);
  * a=workgroups of LDS variant `index` resident at once on the device (the
  * batch's persistent launch shape), b=its LDS bytes per workgroup). */
 int se_world_set_variant(se_world* w, int32_t plain_variant, int32_t lds_variant);
+/* Tile sleeping of se_world_step_blocked (no reference counterpart; on by
+ * default): an LDS tile whose cells, halo included, all hold one material whose
+ * uniform 2x2 block no rule changes skips its sub-steps, and when it and its
+ * eight neighbours were already so in the previous launch it also skips its
+ * loads and stores (its output buffer already holds those cells).  Results are
+ * identical either way; any other write to the grids restarts the record. */
+int se_world_set_sleep(se_world* w, int enable);
 int se_variant_info(const se_world* w, int32_t kind, int32_t index, int32_t* a, int32_t* b, int32_t* c);
 /* Render side-outputs of every step: the colour and light images that
  * setCell writes beside the grid (shaders/compute/operations.glsl:99-170;

@@ -120,6 +120,7 @@ def lib():
                                            ctypes.POINTER(c_u32), ctypes.c_int, c_u32, c_u32]
         L.se_world_step_timed.argtypes = [c_p, c_u32, ctypes.c_int, c_u32, ctypes.POINTER(ctypes.c_float)]
         L.se_world_set_variant.argtypes = [c_p, c_i32, c_i32]
+        L.se_world_set_sleep.argtypes = [c_p, ctypes.c_int]
         L.se_world_copy_timed.argtypes = [c_p, c_u32, ctypes.POINTER(ctypes.c_float)]
         L.se_serve_create.argtypes = [c_p, ctypes.POINTER(c_p), ctypes.c_int, c_p, PP]
         L.se_serve_destroy.argtypes = [c_p]
@@ -377,6 +378,10 @@ class World:
 
     def set_variant(self, plain: int = 3, lds: int = -1):
         _check(lib().se_world_set_variant(self._p, plain, lds))
+
+    def set_sleep(self, enable: bool = True):
+        """Tile sleeping of step_blocked (default on; results are the same)."""
+        _check(lib().se_world_set_sleep(self._p, 1 if enable else 0))
 
     def variant_info(self, kind: int, index: int) -> Tuple[int, int, int]:
         a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()

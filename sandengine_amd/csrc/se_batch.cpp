@@ -171,6 +171,8 @@ int enqueue_plan(Plan& P, hipStream_t s_small, hipStream_t s_large) {
 }
 
 void finish_plan(Plan& P) {
+    for (auto& it : P.small) se_rt::sleep_reset(it.first);
+    for (auto& it : P.large) se_rt::sleep_reset(it.first);
     for (auto& it : P.small) {
         it.first->frame += (int32_t)it.second;
         it.first->cur ^= 1;

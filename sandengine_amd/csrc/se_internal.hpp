@@ -128,12 +128,24 @@ struct se_world {
     float* noise = nullptr;                // colour noise per cell (position only), filled once
     void* color = nullptr;
     int lcur = 0;
+    // tile sleeping of the LDS kernel (se_tile_sleeps): per tile of variant
+    // sleep_variant, the settled code in the last launch or 0xff; valid only while
+    // nothing but that kernel has written either grid buffer (sleep_variant -1
+    // otherwise)
+    unsigned char* d_sleep[2] = {nullptr, nullptr};
+    size_t sleep_cap = 0;
+    int sleep_variant = -1;
+    int sleep_p = 0;
+    bool sleep_enabled = true;
 };
 
 
 namespace se_rt {
 // One Simulation::run of `w` with the plain kernel (frame += 1, mods, step).
 int plain_step(se_world* w);
+// The grids of `w` were written by something other than the LDS kernel: its
+// tile-sleeping record no longer describes them.
+inline void sleep_reset(se_world* w) { w->sleep_variant = -1; }
 // Index of the default LDS tile variant of a module (batches and serving use it
 // unless tuned).
 int default_lds_index(const se_module* m);

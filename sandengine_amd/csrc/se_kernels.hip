@@ -726,7 +726,8 @@ template <int TW, int TH, int HB, int NT>
 __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __restrict__ out, int W, int H,
                                             int pitch, int frame0, int nsub, int tiles_x, int tile,
                                             const se_mod* __restrict__ mods, int nmods,
-                                            const u32* __restrict__ ltbl, unsigned char* lds, u32* flag, u32 seq) {
+                                            const u32* __restrict__ ltbl, unsigned char* lds, u32* flag, u32 seq,
+                                            unsigned char* __restrict__ sleep_out) {
     constexpr int LW = TW + 2 * HB, LH = TH + 2 * HB, LQ = LW / 4;
     // The thread index passes through an empty asm so that per-thread offsets of
     // the tile load/store are recomputed per tile: hoisted out of a persistent
@@ -876,6 +877,7 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     const u32 cref = ref & 0xffu;
     const bool settled = interior && flag[1] != seq && cref < 32u && ((se_stable_mask >> cref) & 1u) != 0u &&
                          (nmods == 0 || !se_mods_touch(mods, nmods, gx0, gx0 + LW - 1, gy0, gy0 + LH - 1));
+    if (sleep_out != nullptr && tid == 0) sleep_out[tile] = settled ? (unsigned char)cref : (unsigned char)0xffu;
     // edge tiles need the WALL restoring of their block loop unless the full
     // table keeps WALL cells by itself
 #if SE_TBL == 1
@@ -1118,6 +1120,57 @@ __device__ __forceinline__ int se_xcd_tile(int b, int n) {
     return (b & 7) * q + (b >> 3);
 }
 
+// One world's tiles: the contiguous runs of se_xcd_tile taken over the tile
+// rows in the order 0, 8, 16, ..., 1, 9, 17, ..., so that each XCD walks rows
+// spread over the whole world (row j, j + 8, ...; neighbouring tiles of a row
+// stay on one XCD).  Work that is uneven in y -- a settled floor, an empty sky
+// -- then spreads evenly over the XCDs instead of leaving some of them idle.
+__device__ __forceinline__ int se_world_tile(int b, int n, int tiles_x) {
+    const int idx = se_xcd_tile(b, n);
+    const int tiles_y = n / tiles_x, p = idx / tiles_x, col = idx - p * tiles_x;
+    int first = 0, row = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {  // rows r = j mod 8 take positions [first, first + count)
+        const int count = (tiles_y - j + 7) >> 3;
+        if (p >= first && p < first + count) row = j + 8 * (p - first);
+        first += count;
+    }
+    return row * tiles_x + col;
+}
+
+// Tile sleeping across launches of one world (se_step_lds_*).  sleep_in[t] is
+// the code c when tile t was settled in the previous launch (every cell of its
+// region, halo included, c with a stable uniform block; no brush near), else
+// 0xff.  If t and its eight neighbours were all settled with the same c and no
+// brush of this launch comes near t, then:
+//   * t's region in the input (t's interior and the neighbours' interiors
+//     around it) is still all c, so the sub-steps leave it as it is, and
+//   * the output buffer, which held the previous launch's input, already holds c
+//     in t's interior;
+// so t neither loads nor stores anything: it records c again and returns.
+// Interior tiles only (an edge tile is never settled, so neither it nor a tile
+// beside it sleeps).
+template <int TW, int TH, int HB>
+__device__ __forceinline__ bool se_tile_sleeps(const unsigned char* __restrict__ sleep_in,
+                                               unsigned char* __restrict__ sleep_out, int tile, int tiles_x,
+                                               int ntiles, const se_mod* __restrict__ mods, int nmods) {
+    if (sleep_in == nullptr) return false;
+    const u32 c = sleep_in[tile];
+    if (c == 0xffu) return false;
+    const int tx = tile % tiles_x, ty = tile / tiles_x, tiles_y = ntiles / tiles_x;
+    if (tx == 0 || ty == 0 || tx == tiles_x - 1 || ty == tiles_y - 1) return false;
+    bool same = true;
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) same = same && sleep_in[tile + dy * tiles_x + dx] == c;
+    if (!same) return false;
+    const int gx0 = tx * TW - HB, gy0 = ty * TH - HB;
+    if (nmods > 0 && se_mods_touch(mods, nmods, gx0, gx0 + TW + 2 * HB - 1, gy0, gy0 + TH + 2 * HB - 1)) return false;
+    if (threadIdx.x == 0) sleep_out[tile] = (unsigned char)c;
+    return true;
+}
+
 // LDS (static, so every table offset is an immediate): tile bytes rounded to 16,
 // the table (SE_TBL_WORDS u32), a 16-byte slot holding the tile's flag words
 // (NULL/WALL seen, mixed tile)
@@ -1131,14 +1184,16 @@ __device__ __forceinline__ int se_xcd_tile(int b, int n) {
 #define SE_LDS_KERNEL(TW, TH, HB, NT)                                                                         \
     extern "C" __global__ __launch_bounds__(NT) void se_step_lds_##TW##x##TH##_h##HB##_t##NT(                \
         const u32* __restrict__ in, u32* __restrict__ out, int W, int H, int pitch, int frame0, int nsub,     \
-        int tiles_x, const se_mod* __restrict__ mods, int nmods, const u32* __restrict__ tbl) {               \
+        int tiles_x, const se_mod* __restrict__ mods, int nmods, const u32* __restrict__ tbl,                 \
+        const unsigned char* __restrict__ sleep_in, unsigned char* __restrict__ sleep_out) {                  \
         constexpr int LT = ((TW + 2 * HB) * (TH + 2 * HB) + 15) & ~15;                                         \
         __shared__ __align__(16) unsigned char se_lds[LT + 4 * SE_TBL_WORDS + 16];                            \
+        const int tile = se_world_tile((int)blockIdx.x, (int)gridDim.x, tiles_x);                             \
+        if (se_tile_sleeps<TW, TH, HB>(sleep_in, sleep_out, tile, tiles_x, (int)gridDim.x, mods, nmods)) return; \
         u32* fl = reinterpret_cast<u32*>(se_lds + LT + 4 * SE_TBL_WORDS);                                      \
         const u32* lt = se_stage_table(tbl, se_lds, (TW + 2 * HB) * (TH + 2 * HB), NT, fl);                  \
-        se_lds_body<TW, TH, HB, NT>(in, out, W, H, pitch, frame0, nsub, tiles_x,                               \
-                                    se_xcd_tile((int)blockIdx.x, (int)gridDim.x), mods, nmods, lt, se_lds, fl, \
-                                    1u);                                                                      \
+        se_lds_body<TW, TH, HB, NT>(in, out, W, H, pitch, frame0, nsub, tiles_x, tile, mods, nmods, lt, se_lds, \
+                                    fl, 1u, sleep_out);                                                       \
     }                                                                                                         \
     extern "C" __global__ __launch_bounds__(NT) void se_batch_lds_##TW##x##TH##_h##HB##_t##NT(               \
         const se_bworld* __restrict__ worlds, const int* __restrict__ prefix, int n_active, int round, int k, \
@@ -1157,7 +1212,7 @@ __device__ __forceinline__ int se_xcd_tile(int b, int n) {
             const int cur = (B.cur0 + round) & 1;                                                             \
             se_lds_body<TW, TH, HB, NT>(B.buf[cur], B.buf[cur ^ 1], B.W, B.H, B.pitch, B.frame0 + done, nsub,  \
                                         B.tiles_x, item - prefix[w], done == 0 ? B.mods : nullptr,            \
-                                        done == 0 ? B.nmods : 0, lt, se_lds, fl, ++seq);                      \
+                                        done == 0 ? B.nmods : 0, lt, se_lds, fl, ++seq, nullptr);             \
         }                                                                                                     \
     }
 

@@ -468,6 +468,7 @@ int render_step(se_world* w, const uint32_t* prev, const uint32_t* cur) {
 
 // One Simulation::run with the plain kernel.
 int se_rt::plain_step(se_world* w) {
+    se_rt::sleep_reset(w);
     w->frame += 1;
     uint32_t* in = w->buf[w->cur];
     uint32_t* out = w->buf[w->cur ^ 1];
@@ -532,16 +533,42 @@ const std::pair<se::LdsVariant, hipFunction_t>* pick_lds(se_world* w, uint32_t s
 }
 
 // Advances `n` <= hb frames with one LDS launch (mods, if any, at its first).
+// With tile sleeping on, the launch records which tiles ended settled and, when
+// the previous launch of the same variant left such a record, lets tiles that
+// stayed settled skip their loads and stores (se_tile_sleeps).
 int lds_launch(se_world* w, const std::pair<se::LdsVariant, hipFunction_t>& v, int n) {
     ModsArg m;
     int rc = take_mods(w, m);
     if (rc) return rc;
     const se::LdsVariant& L = v.first;
     int tiles_x = (int)((w->W + L.tw - 1) / L.tw), tiles_y = (int)((w->H + L.th - 1) / L.th);
+    const int vi = (int)(&v - w->mod->lds.data());
+    const unsigned char* s_in = nullptr;
+    unsigned char* s_out = nullptr;
+    if (w->sleep_enabled) {
+        const size_t ntiles = (size_t)tiles_x * tiles_y;
+        if (ntiles > w->sleep_cap) {
+            hipFree(w->d_sleep[0]);
+            hipFree(w->d_sleep[1]);
+            w->d_sleep[0] = w->d_sleep[1] = nullptr;
+            w->sleep_cap = 0;
+            w->sleep_variant = -1;
+            HIPCHECK(hipMalloc(&w->d_sleep[0], ntiles));
+            HIPCHECK(hipMalloc(&w->d_sleep[1], ntiles));
+            w->sleep_cap = ntiles;
+        }
+        if (w->sleep_variant == vi) s_in = w->d_sleep[w->sleep_p];
+        s_out = w->d_sleep[w->sleep_p ^ 1];
+    }
     rc = launch_ex(v.second, (unsigned)(tiles_x * tiles_y), (unsigned)L.nt, 0u, w->stream,
                    (const uint32_t*)w->buf[w->cur], w->buf[w->cur ^ 1], (int)w->W, (int)w->H, (int)w->pitch,
-                   (int)(w->frame + 1), n, tiles_x, (const void*)m.ptr, m.n, (const uint32_t*)w->mod->table);
+                   (int)(w->frame + 1), n, tiles_x, (const void*)m.ptr, m.n, (const uint32_t*)w->mod->table, s_in,
+                   s_out);
     if (rc) return rc;
+    if (s_out) {
+        w->sleep_p ^= 1;
+        w->sleep_variant = vi;
+    }
     w->frame += n;
     w->cur ^= 1;
     return SE_OK;
@@ -796,6 +823,8 @@ void se_world_destroy(se_world* w) {
         hipFree(w->light[1]);
         hipFree(w->color);
         hipFree(w->noise);
+        hipFree(w->d_sleep[0]);
+        hipFree(w->d_sleep[1]);
     }
     delete w;
 }
@@ -804,6 +833,7 @@ int se_world_upload(se_world* w, const uint32_t* cells, size_t n) {
     if (!w || !cells) return fail(SE_ERR_ARG, "null argument");
     if (n != (size_t)w->W * w->H) return fail(SE_ERR_ARG, "upload size mismatch");
     DeviceGuard dg(w->ctx->device);
+    se_rt::sleep_reset(w);
     HIPCHECK(hipMemcpy2DAsync(w->buf[w->cur], (size_t)w->pitch * 4, cells, (size_t)w->W * 4, (size_t)w->W * 4, w->H,
                               hipMemcpyHostToDevice, w->stream));
     HIPCHECK(hipStreamSynchronize(w->stream));
@@ -852,6 +882,14 @@ int se_world_set_variant(se_world* w, int32_t plain_variant, int32_t lds_variant
     if (lds_variant < -1 || lds_variant >= (int)w->mod->lds.size()) return fail(SE_ERR_ARG, "lds variant out of range");
     w->plain_variant = plain_variant;
     w->lds_variant = lds_variant;
+    se_rt::sleep_reset(w);
+    return SE_OK;
+}
+
+int se_world_set_sleep(se_world* w, int enable) {
+    if (!w) return fail(SE_ERR_ARG, "null world");
+    w->sleep_enabled = enable != 0;
+    se_rt::sleep_reset(w);
     return SE_OK;
 }
 
@@ -1083,6 +1121,7 @@ int se_world_sync(se_world* w) {
 
 int se_world_device_ptr(se_world* w, void** ptr, uint32_t* pitch) {
     if (!w) return fail(SE_ERR_ARG, "null world");
+    se_rt::sleep_reset(w);  // the caller may write through the pointer
     if (ptr) *ptr = w->buf[w->cur];
     if (pitch) *pitch = w->pitch;
     return SE_OK;
@@ -1092,6 +1131,7 @@ int se_world_fill_seeded(se_world* w, uint64_t seed, const uint32_t* ids, const 
                          uint32_t stone_rows, uint32_t stone_id) {
     if (!w || !ids || !thresholds || n < 1 || n > 64) return fail(SE_ERR_ARG, "bad fill arguments");
     DeviceGuard dg(w->ctx->device);
+    se_rt::sleep_reset(w);
     // ids and thresholds are staged in the world's modification buffer (8 KiB)
     void* d = w->d_mods;
     uint32_t host[128] = {0};
@@ -1112,6 +1152,7 @@ int se_world_fill_seeded(se_world* w, uint64_t seed, const uint32_t* ids, const 
 int se_world_copy_timed(se_world* w, uint32_t reps, float* gpu_ms) {
     if (!w || !gpu_ms) return fail(SE_ERR_ARG, "null argument");
     DeviceGuard dg(w->ctx->device);
+    se_rt::sleep_reset(w);
     size_t n = (size_t)w->pitch * w->H;
     if (n % 4) return fail(SE_ERR_ARG, "grid size not a multiple of 4 cells");
     size_t n4 = n / 4;
