@@ -204,17 +204,23 @@ def run_batch(args):
     for (side, rs, steps, _), ws in zip(specs, slow):
         bs.add(ws, steps)
     t0 = time.perf_counter()
-    bf.autotune(rounds=args.tune_rounds)
-    tune_s = time.perf_counter() - t0
     tuned_by_rules = {}
+    if args.tune_rounds > 0:
+        bf.autotune(rounds=args.tune_rounds)
+    tune_s = time.perf_counter() - t0
     for name, rs in rsets.items():
-        vi, k, persistent = bf.tuned(rs)
         w0 = next(w for (_, r, _, _), w in zip(specs, fast) if r == name)
+        if args.tune_rounds > 0:
+            vi, k, persistent = bf.tuned(rs)
+        else:  # the defaults se_batch_run uses untuned
+            vi, k, persistent = w0.variant_info(2, 0)[0], 0, False
         info = w0.variant_info(1, vi)
         tuned_by_rules[name] = {"variant": vi, "tile": [info[0], info[1]], "halo": info[2] // 1000,
-                                "threads": info[2] % 1000, "k": k,
-                                "launch": "persistent" if persistent else "one workgroup per work item"}
-        log(f"autotune {name}: variant {vi} {info} k={k} persistent={persistent}")
+                                "threads": info[2] % 1000, "k": k or info[2] // 1000,
+                                "launch": "persistent" if persistent else "one workgroup per work item",
+                                "tuned": args.tune_rounds > 0}
+        log(f"{'autotune' if args.tune_rounds > 0 else 'default'} {name}: variant {vi} {info} k={k} "
+            f"persistent={persistent}")
     log(f"autotune: {tune_s:.1f}s")
     t0 = time.perf_counter()
     ms_batch = bf.run(mode=1, graph=True)
@@ -236,7 +242,7 @@ def run_batch(args):
     nb = min(64, len(specs))
     cpu = None if args.no_cpu_baseline else cpu_baseline(
         [(rs, side, fseed, BATCH_FILLS[rs][0], BATCH_FILLS[rs][1], 0, 0) for side, rs, _, fseed in specs[:nb]],
-        what=f"the first {nb} worlds of the same draw, one after another (one Simulation per world, as the "
+        steps=24, what=f"the first {nb} worlds of the same draw, one after another (one Simulation per world, as the "
              f"reference runs them)")
     out = {
         "metric": "Gcell-updates/s & HBM-roofline %; p50/p99 step latency, batched vs plain",
@@ -307,8 +313,9 @@ def run_render(args):
     prof = load_profile(kernel, args.workload)
     achieved = RENDER_BYTES_PER_CELL * cells / (render_ms / 1e3) / 1e9
     cpu = None if args.no_cpu_baseline else cpu_baseline(
-        single_sample("single16k"), what="1024x1024 world of the same fill and rules as single16k (the reference "
-                                         "program computes colour and light in every step too)")
+        single_sample("single16k"), steps=40,
+        what="2048x2048 world of the same fill and rules as single16k (the reference program computes colour and "
+             "light in every step too)")
     out = {
         "metric": "Gcell-updates/s & HBM-roofline %; p50/p99 step latency, batched vs plain",
         "value": cells * args.steps / wall / 1e9, "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -441,7 +448,7 @@ print(json.dumps(dict(value=cells / dt / 1e9, unit='Gcell-updates/s', cores=thre
     return out
 
 
-def single_sample(wl, side=1024):
+def single_sample(wl, side=2048):
     """A side x side world of the same rules and fill as workload `wl`."""
     _, rsname, seed, ids, pct, (num, den), stone = WORKLOADS[wl]
     return [(rsname, side, seed, ids, pct, (side * num) // den, stone)]
@@ -489,7 +496,7 @@ def main():
     ap.add_argument("--verify", type=int, default=0, help="serve: replay every n-th request alone (0: none)")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--sub", type=int, default=8, help="sub-steps per LDS launch")
-    ap.add_argument("--paths", default="plain,blocked")
+    ap.add_argument("--paths", default="plain,blocked,blocked_dense")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-looped", action="store_true", help="batch: skip the looped plain-kernel comparison")
@@ -526,7 +533,9 @@ def main():
     lds_kernel = None
     for mode in args.paths.split(","):
         w = make_world(S, ctx, rs, args.workload)
-        if mode == "blocked":
+        if mode == "blocked_dense":  # the LDS kernel with tile sleeping off: every tile loaded and stored
+            w.set_sleep(False)
+        if mode.startswith("blocked"):
             tw, th, c = w.variant_info(1, w.variant_info(2, args.sub)[0])
             lds_kernel = f"se_step_lds_{tw}x{th}_h{c // 1000}_t{c % 1000}"
         wall, gpu_ms = time_path(S, w, mode, args.steps, args.warmup, args.sub)
@@ -538,7 +547,7 @@ def main():
                  kernel="se_step_plain_r1_nt1" if mode == "plain" else lds_kernel)
         # roofline timing: launches that each run the full `sub` sub-steps (the timed
         # steps may end on a shorter launch), between HIP events on the world's stream
-        if mode == "blocked":
+        if mode.startswith("blocked"):
             w.step_timed(2 * args.sub, 1, args.sub)
             full_ms = w.step_timed(args.sub * args.roofline_launches, 1, args.sub) / args.roofline_launches
         else:
@@ -555,21 +564,26 @@ def main():
         results[mode] = r
         worlds[mode] = w
     check = None
-    if not args.no_check and "plain" in worlds and "blocked" in worlds:
-        # both advanced warmup + steps + latency steps from the same grid; align frames
-        a, b = worlds["plain"], worlds["blocked"]
-        if a.frame < b.frame:
-            a.step(b.frame - a.frame)
-        elif b.frame < a.frame:
-            b.step(a.frame - b.frame)
+    if not args.no_check and "plain" in worlds and len(worlds) > 1:
+        # every path advanced warmup + steps + latency steps from the same grid;
+        # align the frames and compare each blocked path's grid with the plain one's
+        a = worlds["plain"]
+        top = max(w.frame for w in worlds.values())
+        for w in worlds.values():
+            if w.frame < top:
+                w.step(top - w.frame)
         ga = a.download()
-        gb = b.download()
-        check = dict(identical=bool(np.array_equal(ga, gb)), frame=a.frame)
-        del ga, gb
-        log(f"plain vs blocked after frame {a.frame}: identical={check['identical']}")
+        check = {"frame": top, "identical": True}
+        for mode, w in worlds.items():
+            if mode != "plain":
+                same = bool(np.array_equal(ga, w.download()))
+                check[f"{mode}_equals_plain"] = same
+                check["identical"] = check["identical"] and same
+        del ga
+        log(f"blocked paths vs plain after frame {top}: {check}")
     for w in worlds.values():
         w.close()
-    best = max(results, key=lambda m: results[m]["gcups"])
+    best = max((m for m in results if m in ("plain", "blocked")), key=lambda m: results[m]["gcups"])
     R = results[best]
     kernel = "se_step_plain_r1_nt1" if best == "plain" else lds_kernel
     prof = load_profile(kernel, args.workload)
@@ -582,7 +596,8 @@ def main():
                 "instr_per_launch": prof["valu_wave_instr_per_launch"],
                 "source": "SQ_INSTS_VALU per launch from profiles/traffic.json over this run's launch time"}
     cpu = None if args.no_cpu_baseline else cpu_baseline(
-        single_sample(args.workload), what=f"1024x1024 world of the same fill and rules as {args.workload}")
+        single_sample(args.workload), steps=40,
+        what=f"2048x2048 world of the same fill and rules as {args.workload} (llvmpipe holds at most 8192^2 RGBA32F)")
     out = {
         "metric": "Gcell-updates/s & HBM-roofline %; p50/p99 step latency, batched vs plain",
         "value": R["gcups"],
@@ -604,7 +619,9 @@ def main():
                      "launch_timing": ("the timed steps' launches" if best == "plain" else
                                        f"{args.roofline_launches} launches of {args.sub} sub-steps after the timed "
                                        f"steps"),
-                     "binding_limit": "hbm" if best == "plain" else "valu issue and lds latency (see roofline_valu)"},
+                     "binding_limit": "hbm" if best == "plain" else
+                     ("hbm: the tile loads and stores of the tiles that do not sleep (measured traffic per launch "
+                      "in `traffic`), with the LDS sub-steps partly hidden under them")},
         "roofline_valu": valu,
         "cpu_baseline": cpu,
         "copy_reference": {"gbs": copy_gbs, "ms": copy_ms, "frac_of_peak": copy_gbs / HBM_PEAK_GBS},

@@ -746,11 +746,11 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     // whole tile inside the world, workgroup a whole number of tile rows wide:
     // each thread keeps one quad column and steps down the rows by pointer
     // increments (no per-quad index arithmetic), U loads in flight before any is
-    // decoded into LDS
+    // decoded into LDS (the whole tile at once up to 16 per thread: +1.5 % over 8)
     constexpr int DROW = NT % LQ == 0 ? NT / LQ : 1, KMAX = LH / DROW;
     constexpr bool ROWS = NT % LQ == 0 && LH % DROW == 0 && KMAX % 8 == 0;
     if (ROWS && interior) {
-        constexpr int U = 8;
+        constexpr int U = KMAX <= 16 ? KMAX : 8;
         const int lq = tid % LQ, ly0 = tid / LQ;
         const u32* src = in + (size_t)(gy0 + ly0) * pitch + gx0 + 4 * lq;
         const size_t sstep = (size_t)DROW * pitch;
