@@ -43,6 +43,29 @@ struct DevArray {
     }
 };
 
+// Events, a captured graph and its executable, released on every exit path.
+struct RunResources {
+    std::vector<hipEvent_t> events;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipStream_t> streams;  // drained first: launches may still read the plans' arrays
+    RunResources() = default;
+    RunResources(const RunResources&) = delete;
+    RunResources& operator=(const RunResources&) = delete;
+    ~RunResources() {
+        for (auto s : streams)
+            if (s) (void)hipStreamSynchronize(s);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto e : events) (void)hipEventDestroy(e);
+    }
+    int event(hipEvent_t* out, unsigned flags) {
+        HIPCHECK(hipEventCreateWithFlags(out, flags));
+        events.push_back(*out);
+        return SE_OK;
+    }
+};
+
 }  // namespace
 
 struct se_batch {
@@ -222,13 +245,17 @@ int run_batched(se_batch* b, const std::vector<std::pair<se_world*, uint32_t>>& 
     }
     int rc = ensure_streams(b, 2 * (int)plans.size());
     if (rc) return rc;
-    hipEvent_t e0, e1;
-    HIPCHECK(hipEventCreate(&e0));
-    HIPCHECK(hipEventCreate(&e1));
+    RunResources res;  // declared after `plans`: destroyed (and the stream drained) first
+    res.streams.assign(b->streams, b->streams + 8);
+    res.streams.push_back(ctx->stream);
+    hipEvent_t e0, e1, fork;
     std::vector<hipEvent_t> joins(2 * plans.size());
-    for (auto& j : joins) HIPCHECK(hipEventCreateWithFlags(&j, hipEventDisableTiming));
-    hipEvent_t fork;
-    HIPCHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    rc = res.event(&e0, hipEventDefault);
+    if (!rc) rc = res.event(&e1, hipEventDefault);
+    if (!rc) rc = res.event(&fork, hipEventDisableTiming);
+    for (auto& j : joins)
+        if (!rc) rc = res.event(&j, hipEventDisableTiming);
+    if (rc) return rc;
     auto submit = [&]() -> int {
         HIPCHECK(hipEventRecord(fork, ctx->stream));
         for (size_t i = 0; i < plans.size(); i++) {
@@ -243,19 +270,17 @@ int run_batched(se_batch* b, const std::vector<std::pair<se_world*, uint32_t>>& 
         for (auto& j : joins) HIPCHECK(hipStreamWaitEvent(ctx->stream, j, 0));
         return SE_OK;
     };
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t g = nullptr;
     if (graph) {
         HIPCHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         int r2 = submit();
-        hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &res.graph);
         if (r2) return r2;
         if (ce != hipSuccess) return fail(SE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
-        HIPCHECK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+        HIPCHECK(hipGraphInstantiate(&res.exec, res.graph, nullptr, nullptr, 0));
     }
     HIPCHECK(hipEventRecord(e0, ctx->stream));
     if (graph) {
-        HIPCHECK(hipGraphLaunch(exec, ctx->stream));
+        HIPCHECK(hipGraphLaunch(res.exec, ctx->stream));
     } else {
         rc = submit();
         if (rc) return rc;
@@ -264,12 +289,6 @@ int run_batched(se_batch* b, const std::vector<std::pair<se_world*, uint32_t>>& 
     HIPCHECK(hipEventSynchronize(e1));
     float ms = 0;
     HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (g) (void)hipGraphDestroy(g);
-    for (auto& j : joins) (void)hipEventDestroy(j);
-    (void)hipEventDestroy(fork);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     for (auto& p : plans) finish_plan(*p);
     if (gpu_ms) *gpu_ms = ms;
     return SE_OK;
@@ -329,22 +348,27 @@ int se_batch_run(se_batch* b, int mode, int32_t lds_variant, uint32_t k, int gra
     if (!b) return fail(SE_ERR_ARG, "null batch");
     DeviceGuard dg(b->ctx->device);
     std::vector<std::pair<se_world*, uint32_t>> work = b->items;
+    // every run advances the worlds again: the frame check of se_batch_add holds
+    // for the first run only
+    for (auto& it : work)
+        if ((int64_t)it.first->frame + it.second > INT32_MAX)
+            return fail(SE_ERR_ARG, "frame counter would overflow for a world of the batch");
     if (mode == 0) {  // loop the plain kernel world by world, one launch per step
+        RunResources res;
         hipEvent_t e0, e1;
-        HIPCHECK(hipEventCreate(&e0));
-        HIPCHECK(hipEventCreate(&e1));
+        int rc = res.event(&e0, hipEventDefault);
+        if (!rc) rc = res.event(&e1, hipEventDefault);
+        if (rc) return rc;
         HIPCHECK(hipEventRecord(e0, b->ctx->stream));
         for (auto& it : work)
             for (uint32_t s = 0; s < it.second; s++) {
-                int rc = se_rt::plain_step(it.first);
+                rc = se_rt::plain_step(it.first);
                 if (rc) return rc;
             }
         HIPCHECK(hipEventRecord(e1, b->ctx->stream));
         HIPCHECK(hipEventSynchronize(e1));
         float ms = 0;
         HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         if (gpu_ms) *gpu_ms = ms;
         return SE_OK;
     }
