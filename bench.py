@@ -496,7 +496,7 @@ def main():
     ap.add_argument("--verify", type=int, default=0, help="serve: replay every n-th request alone (0: none)")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--sub", type=int, default=8, help="sub-steps per LDS launch")
-    ap.add_argument("--paths", default="plain,blocked,blocked_dense")
+    ap.add_argument("--paths", default="plain,blocked,blocked_dense,blocked_tuned")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-looped", action="store_true", help="batch: skip the looped plain-kernel comparison")
@@ -530,33 +530,41 @@ def main():
     wc.close()
     copy_gbs = 8.0 * cells / (copy_ms / 1e3) / 1e9
     log(f"copy reference: {copy_ms:.3f} ms per 2x{4 * cells / 2**30:.2f} GiB -> {copy_gbs:.0f} GB/s")
-    lds_kernel = None
     for mode in args.paths.split(","):
         w = make_world(S, ctx, rs, args.workload)
         if mode == "blocked_dense":  # the LDS kernel with tile sleeping off: every tile loaded and stored
             w.set_sleep(False)
+        sub, tune, lds_kernel = args.sub, None, None
+        if mode == "blocked_tuned":  # the variant se_world_autotune picks for this run length (untimed)
+            t0 = time.perf_counter()
+            vi, sub, trial_ms = w.autotune(args.steps, rounds=2)
+            tune = {"variant": vi, "tile": list(w.variant_info(1, vi)[:2]), "halo": sub,
+                    "trial_ms_per_step": trial_ms, "seconds": time.perf_counter() - t0}
+            log(f"autotune for {args.steps} steps: variant {vi} {w.variant_info(1, vi)}, {sub} sub-steps per launch")
         if mode.startswith("blocked"):
-            tw, th, c = w.variant_info(1, w.variant_info(2, args.sub)[0])
+            tw, th, c = w.variant_info(1, w.variant_info(2, sub)[0])
             lds_kernel = f"se_step_lds_{tw}x{th}_h{c // 1000}_t{c % 1000}"
-        wall, gpu_ms = time_path(S, w, mode, args.steps, args.warmup, args.sub)
-        launches = args.steps if mode == "plain" else -(-args.steps // args.sub)
+        wall, gpu_ms = time_path(S, w, mode, args.steps, args.warmup, sub)
+        launches = args.steps if mode == "plain" else -(-args.steps // sub)
         avg_launch_ms = gpu_ms / launches
         r = dict(gcups=cells * args.steps / wall / 1e9, gcups_gpu=cells * args.steps / (gpu_ms / 1e3) / 1e9,
                  ms_per_step=wall * 1e3 / args.steps, avg_launch_ms=avg_launch_ms, launches=launches,
-                 steps_per_launch=1 if mode == "plain" else args.sub,
+                 steps_per_launch=1 if mode == "plain" else sub,
                  kernel="se_step_plain_r1_nt1" if mode == "plain" else lds_kernel)
+        if tune is not None:
+            r["autotune"] = tune
         # roofline timing: launches that each run the full `sub` sub-steps (the timed
         # steps may end on a shorter launch), between HIP events on the world's stream
         if mode.startswith("blocked"):
-            w.step_timed(2 * args.sub, 1, args.sub)
-            full_ms = w.step_timed(args.sub * args.roofline_launches, 1, args.sub) / args.roofline_launches
+            w.step_timed(2 * sub, 1, sub)
+            full_ms = w.step_timed(sub * args.roofline_launches, 1, sub) / args.roofline_launches
         else:
             full_ms = avg_launch_ms
         r["full_launch_ms"] = full_ms
         r["achieved_gbs"] = 8.0 * cells / (full_ms / 1e3) / 1e9
         r["roofline_frac"] = r["achieved_gbs"] / HBM_PEAK_GBS
         r["frac_of_copy"] = r["achieved_gbs"] / copy_gbs
-        p50, p99 = latencies(w, mode, args.sub, args.latency_samples)
+        p50, p99 = latencies(w, mode, sub, args.latency_samples)
         r["step_latency_ms"] = {"p50": p50, "p99": p99}
         log(f"{mode}: {r['gcups']:.1f} Gcell-upd/s wall, {r['gcups_gpu']:.1f} by events, "
             f"{r['avg_launch_ms']:.3f} ms/launch in the timed steps, {full_ms:.3f} ms per full launch, "
@@ -583,9 +591,12 @@ def main():
         log(f"blocked paths vs plain after frame {top}: {check}")
     for w in worlds.values():
         w.close()
-    best = max((m for m in results if m in ("plain", "blocked")), key=lambda m: results[m]["gcups"])
+    # headline: the fastest of the plain kernel, the LDS kernel at the default
+    # variant and sub-steps, and the LDS kernel as se_world_autotune picks it for
+    # this run length (its trials run before the warmup, outside the timed steps)
+    best = max((m for m in results if m in ("plain", "blocked", "blocked_tuned")), key=lambda m: results[m]["gcups"])
     R = results[best]
-    kernel = "se_step_plain_r1_nt1" if best == "plain" else lds_kernel
+    kernel = R["kernel"]
     prof = load_profile(kernel, args.workload)
     traffic = prof.get("bytes_per_launch")
     valu = None
@@ -617,8 +628,8 @@ def main():
                      "unit": "GB/s", "frac": R["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": 8 * cells, "avg_launch_ms": R["full_launch_ms"],
                      "launch_timing": ("the timed steps' launches" if best == "plain" else
-                                       f"{args.roofline_launches} launches of {args.sub} sub-steps after the timed "
-                                       f"steps"),
+                                       f"{args.roofline_launches} launches of {R['steps_per_launch']} sub-steps "
+                                       f"after the timed steps"),
                      "binding_limit": "hbm" if best == "plain" else
                      ("hbm: the tile loads and stores of the tiles that do not sleep (measured traffic per launch "
                       "in `traffic`), with the LDS sub-steps partly hidden under them")},

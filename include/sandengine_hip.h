@@ -184,6 +184,15 @@ int se_world_fill_seeded(se_world* w, uint64_t seed, const uint32_t* ids, const 
 /* Runs nsteps (mode 0 plain, 1 blocked with `sub`) between two HIP events on
  * the world's stream; *gpu_ms receives their elapsed time. */
 int se_world_step_timed(se_world* w, uint32_t nsteps, int mode, uint32_t sub, float* gpu_ms);
+/* No reference counterpart (Simulation::run dispatches one fixed shader,
+ * simulation.rs:175).  Chooses the LDS variant for the world's next `nsteps`
+ * blocked steps by timing min(nsteps, 48) of them with every variant, from a
+ * snapshot of grid, frame and queued modifications that is restored after each
+ * trial; the fastest becomes the world's variant (as se_world_set_variant), and
+ * *best_sub receives its halo, the sub-steps se_world_step_blocked(w, n, 0)
+ * then runs per launch.  Refused while render side-outputs are on. */
+int se_world_autotune(se_world* w, uint32_t nsteps, uint32_t rounds, int32_t* best_variant, uint32_t* best_sub,
+                      float* best_ms_per_step);
 /* `reps` streaming copies of the grid between the world's two buffers (the
  * same bytes a plain step reads and writes, no rules): the measured HBM
  * reference for the plain kernel.  Leaves the grid contents unchanged. */

@@ -120,6 +120,8 @@ def lib():
                                            ctypes.POINTER(c_u32), ctypes.c_int, c_u32, c_u32]
         L.se_world_step_timed.argtypes = [c_p, c_u32, ctypes.c_int, c_u32, ctypes.POINTER(ctypes.c_float)]
         L.se_world_set_variant.argtypes = [c_p, c_i32, c_i32]
+        L.se_world_autotune.argtypes = [c_p, c_u32, c_u32, ctypes.POINTER(c_i32), ctypes.POINTER(c_u32),
+                                        ctypes.POINTER(ctypes.c_float)]
         L.se_world_set_sleep.argtypes = [c_p, ctypes.c_int]
         L.se_world_copy_timed.argtypes = [c_p, c_u32, ctypes.POINTER(ctypes.c_float)]
         L.se_serve_create.argtypes = [c_p, ctypes.POINTER(c_p), ctypes.c_int, c_p, PP]
@@ -378,6 +380,15 @@ class World:
 
     def set_variant(self, plain: int = 3, lds: int = -1):
         _check(lib().se_world_set_variant(self._p, plain, lds))
+
+    def autotune(self, nsteps: int, rounds: int = 2) -> Tuple[int, int, float]:
+        """Times the next min(nsteps, 48) blocked steps with every LDS variant
+        (grid, frame and queued modifications restored after each trial) and keeps
+        the fastest for step_blocked(n, 0).  Returns (variant, sub-steps per
+        launch, ms per step)."""
+        v, k, ms = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_float()
+        _check(lib().se_world_autotune(self._p, nsteps, rounds, ctypes.byref(v), ctypes.byref(k), ctypes.byref(ms)))
+        return v.value, k.value, ms.value
 
     def set_sleep(self, enable: bool = True):
         """Tile sleeping of step_blocked (default on; results are the same)."""

@@ -711,6 +711,11 @@ __device__ __forceinline__ void se_lds_substep(unsigned char* lds, const u32* __
                                                bool fixed) {
     const bool odd = (fx & 1) != 0;  // block origins share the parity of the x offset
     const bool clean = SE_TBL == 2 && !edge && !fixed;
+    if (SE_TBL == 2 && edge && !fixed) {  // edge tile with no NULL or WALL inside the world
+        if (odd) se_lds_substep_t<LW, NT, true, true, false>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
+        else se_lds_substep_t<LW, NT, false, true, false>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
+        return;
+    }
     if (odd) {
         if (edge) se_lds_substep_t<LW, NT, true, true, true>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
         else if (clean) se_lds_substep_t<LW, NT, true, false, false>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
@@ -825,11 +830,16 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                     if (gy >= 0 && gy < H) {
                         if (xfull) {
                             q = se_pack_quad(v[u]);
+                            fixed |= se_has_fixed(q);
                         } else {
                             const u32* row = in + (size_t)gy * pitch;
                             u32 c[4];
 #pragma unroll
-                            for (int j = 0; j < 4; j++) c[j] = gx + j >= 0 && gx + j < W ? se_enc(row[gx + j]) : SE_LWALL;
+                            for (int j = 0; j < 4; j++) {
+                                const bool inw = gx + j >= 0 && gx + j < W;
+                                c[j] = inw ? se_enc(row[gx + j]) : SE_LWALL;
+                                fixed |= inw && se_has_fixed(c[j]);
+                            }
                             q = se_pack4(c[0], c[1], c[2], c[3]);
                         }
                     }
@@ -852,11 +862,12 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                 if (gx >= 0 && gx + 3 < W) {
                     const uint4 v = *reinterpret_cast<const uint4*>(row + gx);
                     v0 = se_enc(v.x); v1 = se_enc(v.y); v2 = se_enc(v.z); v3 = se_enc(v.w);
+                    fixed |= se_has_fixed(se_pack4(v0, v1, v2, v3));
                 } else {
-                    if (gx >= 0 && gx < W) v0 = se_enc(row[gx]);
-                    if (gx + 1 >= 0 && gx + 1 < W) v1 = se_enc(row[gx + 1]);
-                    if (gx + 2 >= 0 && gx + 2 < W) v2 = se_enc(row[gx + 2]);
-                    if (gx + 3 >= 0 && gx + 3 < W) v3 = se_enc(row[gx + 3]);
+                    if (gx >= 0 && gx < W) { v0 = se_enc(row[gx]); fixed |= se_has_fixed(v0); }
+                    if (gx + 1 >= 0 && gx + 1 < W) { v1 = se_enc(row[gx + 1]); fixed |= se_has_fixed(v1); }
+                    if (gx + 2 >= 0 && gx + 2 < W) { v2 = se_enc(row[gx + 2]); fixed |= se_has_fixed(v2); }
+                    if (gx + 3 >= 0 && gx + 3 < W) { v3 = se_enc(row[gx + 3]); fixed |= se_has_fixed(v3); }
                 }
             }
             lds32[ly * LQ + lq] = se_pack4(v0, v1, v2, v3);
@@ -890,8 +901,11 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
         int ox, oy;
         se_offset(f, ox, oy);
         int fx, nbx, fy, nby;
-        se_block_range(s, ox, gx0, LW, W, fx, nbx);
-        se_block_range(s, oy, gy0, LH, H, fy, nby);
+        // with fewer sub-steps than the halo, the outer HB - nsub rings cannot
+        // reach the interior: the ranges shrink as in a tile of halo nsub
+        const int sr = s + (nsub < HB ? HB - nsub : 0);
+        se_block_range(sr, ox, gx0, LW, W, fx, nbx);
+        se_block_range(sr, oy, gy0, LH, H, fy, nby);
         const int nb = nbx * nby;
         const bool use_mods = s == 0 && nmods > 0;
         if (!use_mods) {
@@ -933,8 +947,8 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                     if ((o = se_mod_override(mods, nmods, gx + 1, gy + 1)) >= 0) c3 = (u32)o;
                 }
                 c0 = se_enc_ok(c0); c1 = se_enc_ok(c1); c2 = se_enc_ok(c2); c3 = se_enc_ok(c3);
-                fixed |= se_has_fixed(se_pack4(c0, c1, c2, c3));
                 if (interior) {
+                    fixed |= se_has_fixed(se_pack4(c0, c1, c2, c3));
                     if (ox == 0) {
                         *reinterpret_cast<unsigned short*>(p0) = (unsigned short)(c0 | (c1 << 8));
                         *reinterpret_cast<unsigned short*>(p1) = (unsigned short)(c2 | (c3 << 8));
@@ -945,10 +959,10 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                 } else {
                     const bool inx0 = gx >= 0 && gx < W, inx1 = gx + 1 >= 0 && gx + 1 < W;
                     const bool iny0 = gy >= 0 && gy < H, iny1 = gy + 1 >= 0 && gy + 1 < H;
-                    if (inx0 && iny0) p0[0] = (unsigned char)c0;
-                    if (inx1 && iny0) p0[1] = (unsigned char)c1;
-                    if (inx0 && iny1) p1[0] = (unsigned char)c2;
-                    if (inx1 && iny1) p1[1] = (unsigned char)c3;
+                    if (inx0 && iny0) { p0[0] = (unsigned char)c0; fixed |= se_has_fixed(c0); }
+                    if (inx1 && iny0) { p0[1] = (unsigned char)c1; fixed |= se_has_fixed(c1); }
+                    if (inx0 && iny1) { p1[0] = (unsigned char)c2; fixed |= se_has_fixed(c2); }
+                    if (inx1 && iny1) { p1[1] = (unsigned char)c3; fixed |= se_has_fixed(c3); }
                 }
             }
         }

@@ -52,7 +52,7 @@ constexpr unsigned kMaxLdsBytes = 160 * 1024;        // LDS per workgroup on gfx
 const std::vector<se::LdsVariant>& lds_variants() {
     static const std::vector<se::LdsVariant> v = {
         {112, 112, 8, 256}, {240, 112, 8, 512}, {224, 112, 8, 512}, {96, 112, 8, 256}, {224, 224, 16, 512},
-        {96, 96, 8, 256},   {112, 112, 8, 512}};
+        {96, 96, 8, 256},   {112, 112, 8, 512}, {104, 104, 12, 256}};
     return v;
 }
 
@@ -1195,6 +1195,80 @@ int se_world_step_timed(se_world* w, uint32_t nsteps, int mode, uint32_t sub, fl
     if (rc) return rc;
     if (e != hipSuccess) return fail(SE_ERR_HIP, hipGetErrorString(e));
     *gpu_ms = ms;
+    return SE_OK;
+}
+
+// Picks the LDS variant for the world's next `nsteps` blocked steps by timing
+// them: from a snapshot of the grid, frame and queued modifications (restored
+// after every trial and at the end), each variant advances min(nsteps, 48)
+// steps with sub-steps = its halo (a longer run is timed over whole launches:
+// 48 is a multiple of every generated halo), `rounds` times; the fastest per
+// step becomes the world's variant, as with se_world_set_variant.  Which variant
+// wins depends on the step count: a deeper halo needs fewer launches for a short
+// run (20 steps: two of 10 instead of 8 + 8 + 4) but costs more per step.
+int se_world_autotune(se_world* w, uint32_t nsteps, uint32_t rounds, int32_t* best_variant, uint32_t* best_sub,
+                      float* best_ms_per_step) {
+    if (!w || nsteps == 0 || rounds == 0) return fail(SE_ERR_ARG, "bad arguments");
+    if (w->color) return fail(SE_ERR_ARG, "autotune: render side-outputs are on (blocked steps run the plain kernel)");
+    const uint32_t trial = std::min<uint32_t>(nsteps, 48u);
+    if ((int64_t)w->frame + trial > INT32_MAX) return fail(SE_ERR_ARG, "frame counter would overflow");
+    DeviceGuard dg(w->ctx->device);
+    struct Resources {
+        void* snap = nullptr;
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        ~Resources() {
+            if (snap) (void)hipFree(snap);
+            for (auto e : ev)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } R;
+    const size_t bytes = (size_t)w->pitch * w->H * 4;
+    if (hipMalloc(&R.snap, bytes) != hipSuccess) return fail(SE_ERR_OOM, "autotune snapshot allocation failed");
+    HIPCHECK(hipEventCreate(&R.ev[0]));
+    HIPCHECK(hipEventCreate(&R.ev[1]));
+    HIPCHECK(hipMemcpyAsync(R.snap, w->buf[w->cur], bytes, hipMemcpyDeviceToDevice, w->stream));
+    const int32_t frame0 = w->frame;
+    const int cur0 = w->cur, variant0 = w->lds_variant;
+    const std::vector<se_modification> pending0 = w->pending;
+    auto restore = [&]() -> int {
+        w->frame = frame0;
+        w->cur = cur0;
+        w->pending = pending0;
+        HIPCHECK(hipMemcpyAsync(w->buf[w->cur], R.snap, bytes, hipMemcpyDeviceToDevice, w->stream));
+        HIPCHECK(hipStreamSynchronize(w->stream));
+        se_rt::sleep_reset(w);
+        return SE_OK;
+    };
+    double best = 1e300;
+    int bv = -1;
+    for (uint32_t r = 0; r < rounds; r++) {
+        for (int vi = 0; vi < (int)w->mod->lds.size(); vi++) {
+            w->lds_variant = vi;
+            HIPCHECK(hipEventRecord(R.ev[0], w->stream));
+            int rc = blocked_steps(w, trial, 0);
+            hipError_t e = hipEventRecord(R.ev[1], w->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(R.ev[1]);
+            float ms = 0;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, R.ev[0], R.ev[1]);
+            const int rr = restore();
+            if (rc || rr) {
+                w->lds_variant = variant0;
+                return rc ? rc : rr;
+            }
+            if (e != hipSuccess) {
+                w->lds_variant = variant0;
+                return fail(SE_ERR_HIP, hipGetErrorString(e));
+            }
+            if (ms / trial < best) {
+                best = ms / trial;
+                bv = vi;
+            }
+        }
+    }
+    w->lds_variant = bv;
+    if (best_variant) *best_variant = bv;
+    if (best_sub) *best_sub = (uint32_t)w->mod->lds[bv].first.hb;
+    if (best_ms_per_step) *best_ms_per_step = (float)best;
     return SE_OK;
 }
 

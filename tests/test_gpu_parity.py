@@ -421,3 +421,25 @@ def test_every_kernel_variant_matches_oracle(rsname):
                     w.step_blocked(b - a, sub)
             np.testing.assert_array_equal(w.download(), want, err_msg=f"plain {pv} lds {lv} sub {sub}")
             w.close()
+
+
+@pytest.mark.parametrize("sub", [0, 5])
+def test_coded_table_edge_tiles_against_oracle(sub):
+    """Edge tiles of the coded table run without the NULL/WALL test while no
+    cell inside the world holds either: a 12-material world with no invalid
+    ids, then brushes that place WALL and NULL inside the world near its edges
+    (those tiles must take the tested path from the brush's sub-step on, and
+    from the next launch's load), against the oracle."""
+    print(f"\n[gpu] coded table edge tiles sub={sub}", flush=True)
+    rs = ruleset("twelve")
+    blob = O.compile_ruleset(L.parse_string(load.ruleset_text("twelve")))
+    W, H, steps, first = 530, 301, 30, 6
+    g0 = C.make_input(dict(ruleset="twelve", W=W, H=H, seed=91))
+    mods_at = {9: [[20, 20, 0, 6, 2], [510, 280, 1, 5, 1], [265, 4, 0, 3, 2]],
+               18: [[25, 22, 0, 9, 0], [300, 150, 0, 40, 14]]}
+    want = g0
+    for s in range(steps):
+        want = O.step(blob, want, first + s, mods_at.get(s))
+    got, _, frame = run_protocol(rs, g0, first, steps, mods_at, "blocked", sub)
+    assert frame == first - 1 + steps
+    np.testing.assert_array_equal(got, want)
