@@ -463,6 +463,34 @@ __device__ __forceinline__ void se_blk_write(unsigned char* p0, u32 e) {
     }
 }
 
+// Odd x offset in an interior tile with LDS rows of 128 bytes: a block's cell
+// pairs start at odd columns, so instead of four byte reads each lane reads the
+// 2-byte aligned pairs at p0 - 1 of both rows and takes the next lane's pairs
+// by a DPP wave shift (lane i gets lane i + 1), and writes the aligned pairs at
+// p0 - 1 from its own new block and the previous lane's (wave shift the other
+// way): [prev c1, c0] and [prev c3, c2].  The pair left of lane 0's block and
+// the one right of the last block lie outside the sub-step's block range, in
+// cells no later sub-step of the launch reads (the range shrinks on every side
+// of an interior tile), so what lands there does not matter.
+#ifndef SE_LDS_DPP
+#define SE_LDS_DPP 1
+#endif
+template <int LW>
+__device__ __forceinline__ u32 se_blk_read_odd_dpp(const unsigned char* p0) {
+    const u32 own = (u32)*reinterpret_cast<const unsigned short*>(p0 - 1) |
+                    ((u32)*reinterpret_cast<const unsigned short*>(p0 - 1 + LW) << 16);
+    const u32 nb = (u32)__builtin_amdgcn_mov_dpp((int)own, 0x130, 0xf, 0xf, true);  // wave_shl:1, 0 past the wave
+    return __builtin_amdgcn_perm(nb, own, 0x06030401u);
+}
+
+template <int LW>
+__device__ __forceinline__ void se_blk_write_odd_dpp(unsigned char* p0, u32 e) {
+    const u32 pe = (u32)__builtin_amdgcn_mov_dpp((int)e, 0x138, 0xf, 0xf, true);  // wave_shr:1
+    const u32 w = __builtin_amdgcn_perm(e, pe, 0x06030401u);
+    *reinterpret_cast<unsigned short*>(p0 - 1) = (unsigned short)w;
+    *reinterpret_cast<unsigned short*>(p0 - 1 + LW) = (unsigned short)(w >> 16);
+}
+
 // In an edge tile, the bytes of cells outside the world (columns by colkeep,
 // rows from gy) are replaced by WALL, which those cells hold in LDS.
 __device__ __forceinline__ u32 se_keep_wall(u32 e, u32 colkeep, int gy, int H) {
@@ -634,7 +662,7 @@ __device__ __forceinline__ void se_lds_border(unsigned char* lds, int fx, int nb
 }
 #endif
 
-template <int LW, int NT, bool ODD, bool EDGE, bool FIXED>
+template <int LW, int NT, bool ODD, bool EDGE, bool FIXED, bool OPEN = true>
 __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* __restrict__ ltbl, int fx, int nbx,
                                                  int fy, int nby, int gx0, int gy0, int W, int H, u32 f) {
     constexpr int NW = NT / 64;
@@ -643,9 +671,15 @@ __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* 
     u32 kf[4];
 #pragma unroll
     for (int l = 0; l < 4; l++) kf[l] = f * se_hash_kf(l);
+    // odd x in an interior tile (OPEN: the block ranges shrink on every side)
+    // with 128-byte LDS rows: aligned pair accesses and DPP shifts instead of
+    // byte accesses (se_blk_read_odd_dpp); the lane after the last block stays to
+    // write that block's right cells (its own block is not evaluated: t = 0)
+    constexpr bool DPP = SE_LDS_DPP && ODD && !EDGE && OPEN && LW == 128;
     for (int bx0 = 0; bx0 < nbx; bx0 += 64) {
         const int bx = bx0 + lane;
-        if (bx >= nbx) break;
+        if (DPP ? bx > nbx : bx >= nbx) break;
+        const bool valid = bx < nbx;
         const int lx = fx + 2 * bx;
         const int gx = gx0 + lx;
         u32 xa[4];
@@ -671,6 +705,20 @@ __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* 
             const bool skipa = colb || (BORDER && (gya < 0 || gya + 1 >= H));
             const bool skipb = colb || (BORDER && (gyb < 0 || gyb + 1 >= H));
             u32 ea, eb;
+            if (DPP) {
+                u32 ta = se_blk_read_odd_dpp<LW>(pa), tb = se_blk_read_odd_dpp<LW>(pb);
+                ta = valid ? ta : 0u;
+                tb = valid ? tb : 0u;
+#if SE_TBL == 0
+                eb = 0u;
+                ea = se_blk_eval1(ta, xa, kf, gya);
+#else
+                se_blk_eval2<FIXED>(ltbl, ta, tb, xa, kf, gya, gyb, false, false, ea, eb);
+#endif
+                se_blk_write_odd_dpp<LW>(pa, ea);
+                if (two) se_blk_write_odd_dpp<LW>(pb, eb);
+                continue;
+            }
 #if SE_TBL == 1
             if (ODD) {  // full table at odd x: offsets from the cell bytes
                 u32 ha[4], hb[4];
@@ -704,11 +752,15 @@ __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* 
 
 // One Margolus sub-step without modifications; odd x offsets, edge tiles and
 // (coded table) interior tiles free of NULL and WALL get their own
-// instantiations.
+// instantiations.  edge: the tile needs the edge-tile block loop; open: the
+// tile lies inside the world (a full table that keeps WALL runs edge tiles
+// through the interior loop, but not through its aligned-pair odd path, which
+// writes cells just outside the block range: in an edge tile those can be the
+// world's WALL border).
 template <int LW, int NT>
 __device__ __forceinline__ void se_lds_substep(unsigned char* lds, const u32* __restrict__ ltbl, int fx, int nbx,
                                                int fy, int nby, int gx0, int gy0, int W, int H, u32 f, bool edge,
-                                               bool fixed) {
+                                               bool fixed, bool open) {
     const bool odd = (fx & 1) != 0;  // block origins share the parity of the x offset
     const bool clean = SE_TBL == 2 && !edge && !fixed;
     if (SE_TBL == 2 && edge && !fixed) {  // edge tile with no NULL or WALL inside the world
@@ -718,6 +770,8 @@ __device__ __forceinline__ void se_lds_substep(unsigned char* lds, const u32* __
     }
     if (odd) {
         if (edge) se_lds_substep_t<LW, NT, true, true, true>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
+        else if (SE_TBL == 1 && !open)
+            se_lds_substep_t<LW, NT, true, false, true, false>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
         else if (clean) se_lds_substep_t<LW, NT, true, false, false>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
         else se_lds_substep_t<LW, NT, true, false, true>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f);
     } else {
@@ -909,7 +963,7 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
         const int nb = nbx * nby;
         const bool use_mods = s == 0 && nmods > 0;
         if (!use_mods) {
-            se_lds_substep<LW, NT>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f, mask_wall, tile_fixed);
+            se_lds_substep<LW, NT>(lds, ltbl, fx, nbx, fy, nby, gx0, gy0, W, H, f, mask_wall, tile_fixed, interior);
         } else if (nb > 0) {
             const int dby = NT / nbx, dbx = NT - dby * nbx;
             int by = tid / nbx, bx = tid - by * nbx;
