@@ -468,6 +468,22 @@ def load_profile(kernel_prefix, workload):
     return {}
 
 
+def binding_limit(path, traffic, launch_ms, valu):
+    """What holds the path's kernel back, from the measured figures: HBM traffic
+    per launch over the launch time against the VALU issue rate."""
+    if path == "plain":
+        return "hbm"
+    hbm = traffic / (launch_ms / 1e3) / (HBM_PEAK_GBS * 1e9) if traffic else None
+    v = valu["frac"] if valu else None
+    if hbm is None or v is None:
+        return "unmeasured: no counter profile for this kernel in profiles/traffic.json"
+    if v >= hbm:
+        return (f"valu: the block loop issues VALU at {v:.2f} of the 614 G/s peak, while the measured HBM traffic "
+                f"(`traffic`) moves at {hbm:.2f} of 8 TB/s")
+    return (f"hbm: the tile loads and stores move the measured traffic (`traffic`) at {hbm:.2f} of 8 TB/s, the "
+            f"block loop issues VALU at {v:.2f} of its peak")
+
+
 # One wave64 VALU instruction per cycle per CU (4 SIMD16 units), 256 CUs, 2.4 GHz: the
 # issue rate measured on the MI355X for the forms the block loop is made of (VOP3, SDWA,
 # 32-bit multiplies, VOP2 with a scalar or constant operand).  VOP2 ops with two VGPR
@@ -630,9 +646,7 @@ def main():
                      "launch_timing": ("the timed steps' launches" if best == "plain" else
                                        f"{args.roofline_launches} launches of {R['steps_per_launch']} sub-steps "
                                        f"after the timed steps"),
-                     "binding_limit": "hbm" if best == "plain" else
-                     ("hbm: the tile loads and stores of the tiles that do not sleep (measured traffic per launch "
-                      "in `traffic`), with the LDS sub-steps partly hidden under them")},
+                     "binding_limit": binding_limit(best, traffic, R["full_launch_ms"], valu)},
         "roofline_valu": valu,
         "cpu_baseline": cpu,
         "copy_reference": {"gbs": copy_gbs, "ms": copy_ms, "frac_of_peak": copy_gbs / HBM_PEAK_GBS},
