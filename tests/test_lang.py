@@ -7,6 +7,7 @@
 * the HIP backend's output compiles with hiprtc for gfx950;
 * the library exports every function declared in include/sandengine_hip.h.
 """
+import ctypes
 import hashlib
 import os
 import re
@@ -169,3 +170,14 @@ def test_hip_backend_picks_the_block_evaluation_form():
     assert twelve["SE_CT_MIRROR_T"] == "0x7fffffbfu"  # rand.x <= 0.5 in the threshold form
     feat = _defines(S.emit_hip(S.bundled_ruleset("features")))
     assert feat["SE_TBL"] == "0" and feat["SE_TBL_WORDS"] == "0u"
+
+
+def test_world_entry_points_refuse_null_handles():
+    """The C ABI answers a null world with SE_ERR_ARG and a message, before any
+    device call (runs without a GPU)."""
+    lib = S.lib()
+    v, k, ms = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_float()
+    assert lib.se_world_autotune(None, 20, 1, ctypes.byref(v), ctypes.byref(k), ctypes.byref(ms)) == S.SE_ERR_ARG
+    assert lib.se_last_error()
+    assert lib.se_world_step_blocked(None, 8, 0) == S.SE_ERR_ARG
+    assert lib.se_world_set_sleep(None, 1) == S.SE_ERR_ARG
