@@ -475,6 +475,11 @@ __device__ __forceinline__ void se_blk_write(unsigned char* p0, u32 e) {
 #ifndef SE_LDS_DPP
 #define SE_LDS_DPP 1
 #endif
+// a compile-time flag passed to a generic lambda
+template <bool B>
+struct se_const_bool {
+    static constexpr bool value = B;
+};
 template <int LW>
 __device__ __forceinline__ u32 se_blk_read_odd_dpp(const unsigned char* p0) {
     const u32 own = (u32)*reinterpret_cast<const unsigned short*>(p0 - 1) |
@@ -695,47 +700,54 @@ __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* 
         // full table: two block rows (by, by + NW) per iteration, their reads,
         // lookups and writes interleaved so the two LDS chains overlap; the coded
         // table and the branchy rule chain run one block row per iteration (each
-        // measured faster that way)
+        // measured faster that way).  Pairs of rows run in one loop, a last single
+        // row after it, so the second row's LDS offset is an immediate.
         constexpr int RPI = SE_TBL == 1 ? 2 : 1;
-        for (int by = wv; by < nby; by += RPI * NW) {
-            const bool two = RPI == 2 && by + NW < nby;  // uniform
+        // hash inputs of lane l for the current block row (x*ka + y*kb + frame*kf),
+        // stepped with the row; the second row's are hx + 2*NW*kb
+        u32 hx[4];
+#pragma unroll
+        for (int l = 0; l < 4; l++) hx[l] = xa[l] + (u32)(gy0 + fy + 2 * wv) * se_hash_kb(l) + kf[l];
+        const u32 kz[4] = {0u, 0u, 0u, 0u};
+        auto row = [&](int by, auto tag) {
+            constexpr bool two = decltype(tag)::value;
             const int gya = gy0 + fy + 2 * by, gyb = gya + 2 * NW;
             unsigned char* pa = lds + (fy + 2 * by) * LW + lx;
-            unsigned char* pb = two ? pa + 2 * NW * LW : pa;
+            unsigned char* pb = pa + 2 * NW * LW;  // read only for a pair of rows
             const bool skipa = colb || (BORDER && (gya < 0 || gya + 1 >= H));
             const bool skipb = colb || (BORDER && (gyb < 0 || gyb + 1 >= H));
             u32 ea, eb;
             if (DPP) {
-                u32 ta = se_blk_read_odd_dpp<LW>(pa), tb = se_blk_read_odd_dpp<LW>(pb);
+                u32 ta = se_blk_read_odd_dpp<LW>(pa), tb = two ? se_blk_read_odd_dpp<LW>(pb) : 0u;
                 ta = valid ? ta : 0u;
                 tb = valid ? tb : 0u;
 #if SE_TBL == 0
                 eb = 0u;
-                ea = se_blk_eval1(ta, xa, kf, gya);
+                ea = se_blk_eval1(ta, hx, kz, 0);
 #else
-                se_blk_eval2<FIXED>(ltbl, ta, tb, xa, kf, gya, gyb, false, false, ea, eb);
+                se_blk_eval2<FIXED>(ltbl, ta, tb, hx, kz, 0, 2 * NW, false, false, ea, eb);
 #endif
                 se_blk_write_odd_dpp<LW>(pa, ea);
                 if (two) se_blk_write_odd_dpp<LW>(pb, eb);
-                continue;
+                return;
             }
 #if SE_TBL == 1
             if (ODD) {  // full table at odd x: offsets from the cell bytes
                 u32 ha[4], hb[4];
-                se_hash_lanes(ha, xa, kf, gya);
-                se_hash_lanes(hb, xa, kf, gyb);
+                se_hash_lanes(ha, hx, kz, 0);
+                se_hash_lanes(hb, hx, kz, 2 * NW);
                 ea = se_tbl_at_bytes<LW>(ltbl, pa, se_rclass(ha[0], ha[1], ha[2], ha[3]));
-                eb = se_tbl_at_bytes<LW>(ltbl, pb, se_rclass(hb[0], hb[1], hb[2], hb[3]));
+                eb = two ? se_tbl_at_bytes<LW>(ltbl, pb, se_rclass(hb[0], hb[1], hb[2], hb[3])) : 0u;
             } else {
-                const u32 ta = se_blk_read<LW, ODD>(pa), tb = se_blk_read<LW, ODD>(pb);
-                se_blk_eval2<FIXED>(ltbl, ta, tb, xa, kf, gya, gyb, skipa, skipb, ea, eb);
+                const u32 ta = se_blk_read<LW, ODD>(pa), tb = two ? se_blk_read<LW, ODD>(pb) : 0u;
+                se_blk_eval2<FIXED>(ltbl, ta, tb, hx, kz, 0, 2 * NW, skipa, skipb, ea, eb);
             }
 #elif SE_TBL == 2
             const u32 ta = se_blk_read<LW, ODD>(pa);
-            se_blk_eval2<FIXED>(ltbl, ta, 0u, xa, kf, gya, gyb, skipa, skipb, ea, eb);
+            se_blk_eval2<FIXED>(ltbl, ta, 0u, hx, kz, 0, 2 * NW, skipa, skipb, ea, eb);
 #else
             eb = 0u;
-            ea = se_blk_eval1(se_blk_read<LW, ODD>(pa), xa, kf, gya);
+            ea = se_blk_eval1(se_blk_read<LW, ODD>(pa), hx, kz, 0);
 #endif
             if (EDGE && !BORDER) {
                 ea = se_keep_wall(ea, colkeep, gya, H);
@@ -743,6 +755,19 @@ __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* 
             }
             if (!skipa) se_blk_write<LW, ODD>(pa, ea);
             if (two && !skipb) se_blk_write<LW, ODD>(pb, eb);
+        };
+        int by = wv;
+        if (RPI == 2) {
+            for (; by + NW < nby; by += 2 * NW) {
+                row(by, se_const_bool<true>());
+#pragma unroll
+                for (int l = 0; l < 4; l++) hx[l] += (u32)(4 * NW) * se_hash_kb(l);
+            }
+        }
+        for (; by < nby; by += NW) {
+            row(by, se_const_bool<false>());
+#pragma unroll
+            for (int l = 0; l < 4; l++) hx[l] += (u32)(2 * NW) * se_hash_kb(l);
         }
     }
 #if SE_TBL == 2
