@@ -269,17 +269,30 @@ extern "C" __global__ __launch_bounds__(256) void se_fill(u32* __restrict__ g, s
 // Cell codes in LDS.  With the coded table (SE_TBL == 2) the LDS tile holds
 // code(id) = (id - 3) mod SE_NMAT: the rule set's materials 3.. become 0..,
 // EMPTY becomes D - 1, NULL D and WALL D + 1 (D = SE_NMAT - 2), so the tuples the
-// table covers are those of codes < D.  Every other path keeps ids in LDS.
+// table covers are those of codes < D.  With the full table (SE_TBL == 1) the
+// LDS tile holds id * SE_CS, SE_CS = 4R (R hash classes) where that is a power
+// of two and keeps every code in a byte: a block's tuple taken over such codes
+// is then the byte offset of its first table entry (one instruction fewer per
+// block than scaling the tuple).  The chain path keeps ids in LDS.
 // se_enc_ok takes an id < SE_NMAT; se_enc decodes any u32 first.
 // ---------------------------------------------------------------------------
+#if SE_TBL == 1
+#define SE_CS_4R (SE_TBL_R * 4u)
+#define SE_CS (((SE_CS_4R & (SE_CS_4R - 1u)) == 0u && SE_CS_4R * (SE_NMAT - 1u) <= 255u) ? SE_CS_4R : 1u)
+#else
+#define SE_CS 1u
+#endif
+// log2 of SE_CS (a power of two)
+#define SE_CS_SHIFT (SE_CS == 1u ? 0u : SE_CS == 2u ? 1u : SE_CS == 4u ? 2u : SE_CS == 8u ? 3u : SE_CS == 16u ? 4u \
+                     : SE_CS == 32u ? 5u : SE_CS == 64u ? 6u : 7u)
 #if SE_TBL == 2
 #define SE_LWALL (SE_NMAT - 1u)
 __device__ __forceinline__ u32 se_enc_ok(u32 id) { return min(id - 3u, id + (SE_NMAT - 3u)); }
 __device__ __forceinline__ u32 se_dec(u32 c) { return min(c + 3u, c - (SE_NMAT - 3u)); }
 #else
-#define SE_LWALL SE_WALLID
-__device__ __forceinline__ u32 se_enc_ok(u32 id) { return id; }
-__device__ __forceinline__ u32 se_dec(u32 c) { return c; }
+#define SE_LWALL (SE_WALLID * SE_CS)
+__device__ __forceinline__ u32 se_enc_ok(u32 id) { return id << SE_CS_SHIFT; }
+__device__ __forceinline__ u32 se_dec(u32 c) { return c >> SE_CS_SHIFT; }
 #endif
 __device__ __forceinline__ u32 se_enc(u32 v) { return se_enc_ok(se_decode(v)); }
 
@@ -295,9 +308,10 @@ __device__ __forceinline__ u32 se_enc(u32 v) { return se_enc_ok(se_decode(v)); }
 extern "C" __device__ u32 se_wall_kept = 0u;
 
 // Bit c set (by the host, from the built table) when a block of four cells of
-// LDS code c comes out unchanged under every hash class and mirror bit.  A tile
-// whose cells all hold such a c is a fixed point of every sub-step: each block
-// evaluated in it holds four cells of c (se_lds_body skips its sub-steps).
+// id c (full table) or LDS code c (coded table) comes out unchanged under every
+// hash class and mirror bit.  A tile whose cells all hold such a c is a fixed
+// point of every sub-step: each block evaluated in it holds four cells of c
+// (se_lds_body skips its sub-steps).
 // 0 (never skip) for rule sets without a table.
 extern "C" __device__ u32 se_stable_mask = 0u;
 
@@ -396,7 +410,7 @@ __device__ __forceinline__ u32 se_pack_quad(uint4 v) {
 #if SE_TBL == 2
     return se_rot4(se_pack4(v.x, v.y, v.z, v.w), SE_NMAT - 3u);  // codes (id - 3) mod SE_NMAT
 #else
-    return se_pack4(v.x, v.y, v.z, v.w);
+    return se_pack4(v.x, v.y, v.z, v.w) << SE_CS_SHIFT;  // id * SE_CS stays in its byte
 #endif
 }
 
@@ -404,6 +418,8 @@ __device__ __forceinline__ u32 se_pack_quad(uint4 v) {
 __device__ __forceinline__ uint4 se_unpack_quad(u32 p) {
 #if SE_TBL == 2
     p = se_rot4(p, 3u);  // ids (code + 3) mod SE_NMAT
+#else
+    p = (p >> SE_CS_SHIFT) & (0x01010101u * (0xffu >> SE_CS_SHIFT));
 #endif
     return make_uint4(p & 0xffu, (p >> 8) & 0xffu, (p >> 16) & 0xffu, p >> 24);
 }
@@ -426,13 +442,15 @@ __device__ __forceinline__ void se_block_range(int s, int par, int g0, int L, in
     cnt = b >= a ? (b - a) / 2 + 1 : 0;
 }
 
-// Tuple index c0 + N*c1 + N^2*c2 + N^3*c3 of a packed block t = c0 | c1<<8 |
-// c2<<16 | c3<<24 (bytes < N <= 255): byte dot products with constant weights.
+// acc + tuple index c0 + N*c1 + N^2*c2 + N^3*c3 of a packed block t = c0 | c1<<8
+// | c2<<16 | c3<<24 (bytes < 256, weights for N <= 255): byte dot products with
+// constant weights.  Linear in the bytes: over codes k * id it gives k * tuple.
 template <u32 N>
-__device__ __forceinline__ u32 se_tuple(u32 t) {
-    if (N <= 6) return __builtin_amdgcn_udot4(t, 1u | (N << 8) | ((N * N) << 16) | ((N * N * N) << 24), 0u, false);
-    if (N <= 15) return __builtin_amdgcn_udot4(t, 1u | (N << 8) | ((N * N) << 16), __umul24(t >> 24, N * N * N), false);
-    const u32 lo = __builtin_amdgcn_udot4(t, 1u | (N << 8), 0u, false);
+__device__ __forceinline__ u32 se_tuple(u32 t, u32 acc = 0u) {
+    if (N <= 6) return __builtin_amdgcn_udot4(t, 1u | (N << 8) | ((N * N) << 16) | ((N * N * N) << 24), acc, false);
+    if (N <= 15)
+        return __builtin_amdgcn_udot4(t, 1u | (N << 8) | ((N * N) << 16), __umul24(t >> 24, N * N * N) + acc, false);
+    const u32 lo = __builtin_amdgcn_udot4(t, 1u | (N << 8), acc, false);
     const u32 hi = __builtin_amdgcn_udot4(t, (1u << 16) | (N << 24), 0u, false);
     return lo + __umul24(hi, N * N);
 }
@@ -564,7 +582,7 @@ __device__ __forceinline__ u32 se_mad24(u32 a, u32 b, u32 c) {
 
 template <int LW>
 __device__ __forceinline__ u32 se_tbl_at_bytes(const u32* __restrict__ ltbl, const unsigned char* p0, u32 r) {
-    constexpr u32 S = SE_TBL_R * 4u, N = SE_NMAT;
+    constexpr u32 S = SE_TBL_R * 4u / SE_CS, N = SE_NMAT;  // codes carry the factor SE_CS
     u32 off = se_mad24(p0[0], S, r * 4u);
     off = se_mad24(p0[1], S * N, off);
     off = se_mad24(p0[LW], S * N * N, off);
@@ -582,12 +600,13 @@ __device__ __forceinline__ void se_blk_eval2(const u32* __restrict__ ltbl, u32 t
     u32 ha[4], hb[4];
     se_hash_lanes(ha, xa, kf, gya);
     se_hash_lanes(hb, xa, kf, gyb);
-    // byte offsets tuple * 4R + 4r (the class term as one select of constants)
+    // byte offsets tuple * 4R + 4r (the class term as one select of constants,
+    // added by the tuple's dot product where the codes carry the factor 4R)
     const unsigned char* lb = reinterpret_cast<const unsigned char*>(ltbl);
-    ea = *reinterpret_cast<const u32*>(lb + se_tuple<SE_NMAT>(ta) * (SE_TBL_R * 4u) +
-                                       se_rclass(ha[0], ha[1], ha[2], ha[3]) * 4u);
-    eb = *reinterpret_cast<const u32*>(lb + se_tuple<SE_NMAT>(tb) * (SE_TBL_R * 4u) +
-                                       se_rclass(hb[0], hb[1], hb[2], hb[3]) * 4u);
+    constexpr u32 TS = SE_TBL_R * 4u / SE_CS;
+    const u32 ra = se_rclass(ha[0], ha[1], ha[2], ha[3]) * 4u, rb = se_rclass(hb[0], hb[1], hb[2], hb[3]) * 4u;
+    ea = *reinterpret_cast<const u32*>(lb + (TS == 1u ? se_tuple<SE_NMAT>(ta, ra) : se_tuple<SE_NMAT>(ta) * TS + ra));
+    eb = *reinterpret_cast<const u32*>(lb + (TS == 1u ? se_tuple<SE_NMAT>(tb, rb) : se_tuple<SE_NMAT>(tb) * TS + rb));
 #elif SE_TBL == 2
     u32 ha[4], hb[4];
     se_hash_lanes(ha, xa, kf, gya);
@@ -761,7 +780,12 @@ __device__ __forceinline__ void se_lds_substep_t(unsigned char* lds, const u32* 
             for (; by + NW < nby; by += 2 * NW) {
                 row(by, se_const_bool<true>());
 #pragma unroll
-                for (int l = 0; l < 4; l++) hx[l] += (u32)(4 * NW) * se_hash_kb(l);
+                for (int l = 0; l < 4; l++) {
+                    hx[l] += (u32)(4 * NW) * se_hash_kb(l);
+                    // one register per lane, stepped by a constant (the compiler
+                    // would otherwise rebuild it from two parts every row)
+                    if (se_hash_used(l)) __asm__("" : "+v"(hx[l]));
+                }
             }
         }
         for (; by < nby; by += NW) {
@@ -964,8 +988,9 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
     // bit c of se_stable_mask set, and no modification near it.  Every block of
     // every sub-step is then four cells of c, which stay as they are: the
     // sub-steps are skipped and the tile is stored as loaded.
-    const u32 cref = ref & 0xffu;
-    const bool settled = interior && flag[1] != seq && cref < 32u && ((se_stable_mask >> cref) & 1u) != 0u &&
+    // (se_stable_mask is indexed by code with the coded table, by id otherwise)
+    const u32 cref = ref & 0xffu, cbit = SE_TBL == 2 ? cref : se_dec(cref);
+    const bool settled = interior && flag[1] != seq && cbit < 32u && ((se_stable_mask >> cbit) & 1u) != 0u &&
                          (nmods == 0 || !se_mods_touch(mods, nmods, gx0, gx0 + LW - 1, gy0, gy0 + LH - 1));
     if (sleep_out != nullptr && tid == 0) sleep_out[tile] = settled ? (unsigned char)cref : (unsigned char)0xffu;
     // edge tiles need the WALL restoring of their block loop unless the full
@@ -1013,8 +1038,9 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
                 const u32 base = (c0 + SE_NMAT * (c1 + SE_NMAT * (c2 + SE_NMAT * c3))) * SE_TBL_R;
                 const u32 r = se_rclass(se_hash_lane(0, (u32)gx, (u32)gy, f), se_hash_lane(1, (u32)gx, (u32)gy, f),
                                         se_hash_lane(2, (u32)gx, (u32)gy, f), se_hash_lane(3, (u32)gx, (u32)gy, f));
-                const u32 e = ltbl[base + r];
-                c0 = e & 0xffu; c1 = (e >> 8) & 0xffu; c2 = (e >> 16) & 0xffu; c3 = e >> 24;
+                const u32 e = ltbl[base + r];  // LDS codes
+                c0 = se_dec(e & 0xffu); c1 = se_dec((e >> 8) & 0xffu); c2 = se_dec((e >> 16) & 0xffu);
+                c3 = se_dec(e >> 24);
 #else
                 se_block(c0, c1, c2, c3, (u32)gx, (u32)gy, f);
 #endif
@@ -1146,7 +1172,9 @@ __device__ __forceinline__ void se_lds_body(const u32* __restrict__ in, u32* __r
 }
 
 // The table (SE_TBL_WORDS u32 words) into LDS after the tile; the two flag
-// words after the table (NULL/WALL seen, mixed tile) start at 0.
+// words after the table (NULL/WALL seen, mixed tile) start at 0.  Full-table
+// entries (ids) become LDS codes (id * SE_CS, each in its own byte).
+__device__ __forceinline__ u32 se_tbl_code(u32 e) { return SE_TBL == 1 ? e << SE_CS_SHIFT : e; }
 __device__ __forceinline__ const u32* se_stage_table(const u32* __restrict__ tbl, unsigned char* lds, int lds_tile,
                                                      int nt, u32* flag) {
     if (threadIdx.x < 2) flag[threadIdx.x] = 0u;
@@ -1167,10 +1195,12 @@ __device__ __forceinline__ const u32* se_stage_table(const u32* __restrict__ tbl
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int i = i0 + u * nt + (int)threadIdx.x;
-            if (i < Q) dst[i] = v[u];
+            const uint4 c = make_uint4(se_tbl_code(v[u].x), se_tbl_code(v[u].y), se_tbl_code(v[u].z),
+                                       se_tbl_code(v[u].w));
+            if (i < Q) dst[i] = c;
         }
     }
-    for (int i = 4 * Q + (int)threadIdx.x; i < tbl_words; i += nt) ltbl[i] = tbl[i];
+    for (int i = 4 * Q + (int)threadIdx.x; i < tbl_words; i += nt) ltbl[i] = se_tbl_code(tbl[i]);
     __syncthreads();
     return ltbl;
 #else
