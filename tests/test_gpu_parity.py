@@ -212,6 +212,44 @@ def test_wall_changing_table_vs_oracle(shape):
         np.testing.assert_array_equal(got, want, err_msg=f"{mode} sub={sub}")
 
 
+def _three_class_rules():
+    """The four-material rule set plus two rules with different probabilities,
+    so hash lane 1 has two thresholds (three classes) beside the mirror lane's
+    two: R = 6 table entries per tuple.  4R = 24 is not a power of two, so the
+    LDS tile keeps plain ids and the table offset is scaled per block (the
+    bundled set, R = 2, keeps ids times 8)."""
+    t = S.bundled_ruleset("four_materials")
+    t2 = t.replace("    extra_rules: [sand_fall]\n", "    extra_rules: [sand_sink, sand_fall]\n", 1)
+    t2 = t2.replace("    extra_rules: [water_flow]\n", "    extra_rules: [water_jitter, water_flow]\n", 1)
+    t2 = t2.replace("types:\n", "  sand_sink:\n    if: DOWN.mat == water\n    probability: 0.3\n"
+                                 "    do: SWAP SELF DOWN\n"
+                                 "  water_jitter:\n    if: isType_EMPTY(DOWNRIGHT)\n    probability: 0.6\n"
+                                 "    do: SWAP SELF DOWNRIGHT\ntypes:\n", 1)
+    assert t2.count("probability") == 2
+    return t2
+
+
+@pytest.mark.parametrize("shape", [(300, 260), (131, 77)], ids=["300x260", "131x77"])
+def test_three_class_table_vs_oracle(shape):
+    """A full table with six hash classes per tuple steps exactly in every kernel."""
+    W, H = shape
+    print(f"\n[gpu] three-class table {W}x{H}", flush=True)
+    text = _three_class_rules()
+    rs = S.RuleSet(ctx(), text)
+    assert rs.kernel_form == "table"
+    blob = O.compile_ruleset(L.parse_string(text))
+    rng = np.random.RandomState(W + 37 * H)
+    g0 = O.fill(int(rng.randint(0, 10 ** 6)), W, H, [0, 3, 4, 5], [45, 25, 20, 10])
+    steps, first = 23, 9
+    mods_at = {5: C._rand_mods(rng, W, H, 6, 3)}
+    want = g0
+    for s in range(steps):
+        want = O.step(blob, want, first + s, mods_at.get(s))
+    for mode, sub in [("plain", 0), ("blocked", 0), ("blocked", 6)]:
+        got, _, _ = run_protocol(rs, g0, first, steps, mods_at, mode, sub)
+        np.testing.assert_array_equal(got, want, err_msg=f"{mode} sub={sub}")
+
+
 def test_clear_frame_and_api_errors():
     rs = ruleset("four")
     w = S.World(ctx(), rs, 40, 30)
