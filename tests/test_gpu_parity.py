@@ -462,6 +462,32 @@ def test_every_kernel_variant_matches_oracle(rsname):
 
 
 @pytest.mark.parametrize("sub", [0, 5])
+def test_full_table_null_tiles_against_oracle(sub):
+    """NULL inside the world under the full table: a four-material world with
+    unknown ids (NULL once simulated) in a block of interior cells and NULL
+    down part of one edge, brushes that place WALL in interior and edge tiles,
+    and a later brush that paints over the NULL block, against the oracle."""
+    print(f"\n[gpu] full table NULL tiles sub={sub}", flush=True)
+    rs = ruleset("four")
+    blob = O.compile_ruleset(L.parse_string(load.ruleset_text("four")))
+    W, H, steps, first = 530, 301, 30, 6
+    g0 = O.fill(77, W, H, [0, 3, 4, 5], [50, 25, 15, 10])
+    g0[200:210, 240:260] = 200     # unknown ids: NULL when their block is simulated
+    g0[0:40, W - 3] = 1            # NULL on the right edge
+    mods_at = {9: [[300, 150, 0, 6, 2], [8, 290, 1, 5, 2], [265, 4, 0, 3, 2]],
+               18: [[300, 150, 0, 9, 0], [250, 205, 1, 30, 3]]}
+    want = g0
+    for s in range(steps):
+        want = O.step(blob, want, first + s, mods_at.get(s))
+        if s == 17:
+            assert (want[200:210, 240:260] == 1).sum() > 0  # the unknown ids became NULL
+    assert (want == 1).sum() == 40 and (want[200:210, 240:260] == 3).all()
+    got, _, frame = run_protocol(rs, g0, first, steps, mods_at, "blocked", sub)
+    assert frame == first - 1 + steps
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("sub", [0, 5])
 def test_coded_table_edge_tiles_against_oracle(sub):
     """Edge tiles of the coded table run without the NULL/WALL test while no
     cell inside the world holds either: a 12-material world with no invalid
