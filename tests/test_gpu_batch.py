@@ -140,3 +140,29 @@ def test_batch_wall_changing_table_against_oracle():
         assert (want == 3).sum() < (g0 == 3).sum()
         np.testing.assert_array_equal(w.download(), want)
     b.close()
+
+
+def test_batch_three_class_table_against_oracle():
+    """A full table with six hash classes per tuple (tile codes left unscaled)
+    in the batched kernel: single-tile worlds and worlds that need rounds, with
+    unknown ids (NULL once simulated) in some of them, against the oracle."""
+    from tests.test_gpu_parity import _three_class_rules
+    text = _three_class_rules()
+    rset = S.RuleSet(ctx(), text)
+    assert rset.kernel_form == "table"
+    blob = O.compile_ruleset(L.parse_string(text))
+    b = S.Batch(ctx())
+    cases = []
+    for i, (W, H, steps) in enumerate([(64, 64, 90), (150, 90, 57), (300, 260, 33), (31, 17, 120), (226, 114, 40)]):
+        g0 = O.fill(90 + i, W, H, [0, 3, 4, 5], [45, 25, 20, 10])
+        if i % 2 == 0:
+            g0[H // 3:H // 3 + 4, W // 4:W // 4 + 6] = 77
+        w = S.World(ctx(), rset, W, H)
+        w.frame = 1
+        w.upload(g0)
+        b.add(w, steps)
+        cases.append((w, g0, steps))
+    b.run(mode=1, graph=True)
+    for w, g0, steps in cases:
+        np.testing.assert_array_equal(w.download(), O.run(blob, g0, 2, steps))
+    b.close()
